@@ -1,0 +1,13 @@
+"""MI355X-native matrix-free DG advection operators and time integrators.
+
+The compute path is libdgops.so (hand-written HIP kernels for gfx950)
+behind the C ABI of include/dgops.h; this package is a thin Python mirror
+of the reference's operator interfaces on top of it.
+"""
+from . import _lib
+from ._lib import DgError
+from .operators import (AdvectionOperation, LowStorageRungeKuttaIntegrator,
+                        irk_coefficients, make_config)
+
+__all__ = ["AdvectionOperation", "LowStorageRungeKuttaIntegrator",
+           "DgError", "make_config", "irk_coefficients", "_lib"]

@@ -1,0 +1,38 @@
+// Host wrappers of the auxiliary kernels: projection, integrals, vector
+// operations and pointwise time-integrator updates.
+#pragma once
+
+#include "internal.hpp"
+
+namespace dgops
+{
+  void aux_project(const Operator &op, void *dst, double t);
+  void aux_mass_energy(const Operator &op, const void *vec, double out[3]);
+  void aux_l2_error(const Operator &op, const void *vec, double t,
+                    double out[2]);
+
+  // per cell: volume average of the reference-space velocity J^-1 a(x)
+  // (at time factor 1) and det(J^-1) at the first quadrature point
+  // (advection_solver_irk_real.cc:1543-1556, :2102)
+  void aux_fdm_cell_data(const Operator &op, double *velocity,
+                         double *invdet);
+
+  // vec = M^-1 vec in place, precision p
+  // (CellwiseInverseMassMatrix::apply, advection_solver_implicit_real.cc:894-909)
+  void aux_inverse_mass(const Operator &op, int p, void *vec);
+
+  // vector operations, precision p (DG_FP64 / DG_FP32)
+  void vec_axpby(Operator &op, int p, int64_t n, double a, const void *x,
+                 double b, void *y);
+  void vec_multi_axpy(Operator &op, int p, int64_t n, int m,
+                      const double *alpha, const void *x, int64_t x_stride,
+                      void *y);
+  void vec_multi_dot(Operator &op, int p, int64_t n, int m, const void *x,
+                     int64_t x_stride, const void *y, double *result);
+  void vec_copy_convert(Operator &op, int64_t n, int ps, const void *src,
+                        int pd, void *dst);
+  // sol_out = sol_in + b k; ri_out = sol_in + a k (ri_out may be null)
+  void vec_lsrk_update(Operator &op, int p, int64_t n, double b, double a,
+                       const void *k, const void *sol_in, void *sol_out,
+                       void *ri_out);
+} // namespace dgops

@@ -1,0 +1,796 @@
+// Auxiliary kernels: L2 projection of analytic data, cell integrals
+// (mass, energy, H1 seminorm, L2 error), vector operations and the
+// pointwise updates of the time integrators. These run once per output or
+// are plain streaming kernels; the hot operator lives in op_kernel.hpp.
+#include "aux.hpp"
+#include "op_kernel.hpp"
+
+namespace dgops
+{
+  namespace
+  {
+    constexpr int aux_threads = 256;
+    constexpr int max_pts     = 729; // (max_n + 1)^3
+
+    struct QuadTables
+    {
+      int    n2;             // points of the second rule
+      double S2[9][max_n];   // nodal basis at the n2-point Gauss rule
+      double w2[9], x2[9];
+    };
+
+    // tensor-product sweep on LDS data: out has extent m along `dir`,
+    // in has extent k along `dir`, other extents given by ext[]
+    __device__ void
+    sweep(const double *M, int ld, const double *in, double *out, int dir,
+          int m, int k, const int (&ext)[3])
+    {
+      int eo[3] = {ext[0], ext[1], ext[2]};
+      int ei[3] = {ext[0], ext[1], ext[2]};
+      eo[dir]   = m;
+      ei[dir]   = k;
+      const int total = eo[0] * eo[1] * eo[2];
+      for (int pt = threadIdx.x; pt < total; pt += blockDim.x)
+        {
+          int q[3] = {pt % eo[0], (pt / eo[0]) % eo[1], pt / (eo[0] * eo[1])};
+          const int qd = q[dir];
+          double    s  = 0;
+          for (int i = 0; i < k; ++i)
+            {
+              q[dir] = i;
+              s += M[qd * ld + i] * in[q[0] + ei[0] * (q[1] + ei[1] * q[2])];
+            }
+          out[pt] = s;
+        }
+    }
+
+    template <typename T>
+    __device__ T
+    load_as(const void *p, int64_t i)
+    {
+      return static_cast<const T *>(p)[i];
+    }
+
+    template <int dim>
+    __global__ void
+    project_kernel(const OpParams P, const DevBasis<double> *__restrict__ B,
+                   int n, int nc, double t, void *dst, int fp32)
+    {
+      __shared__ double a[max_pts], b[max_pts];
+      __shared__ double X[24];
+      constexpr int     NV = 1 << dim;
+      const int64_t     cell = blockIdx.x;
+      for (int e = threadIdx.x; e < NV * dim; e += blockDim.x)
+        X[e] = P.cell_vertices[cell * NV * dim + e];
+      __syncthreads();
+      const int nd     = dim == 3 ? n * n * n : n * n;
+      int       ext[3] = {n, n, dim == 3 ? n : 1};
+      for (int c = 0; c < nc; ++c)
+        {
+          for (int pt = threadIdx.x; pt < nd; pt += blockDim.x)
+            {
+              const int q[3] = {pt % n, (pt / n) % n, dim == 3 ? pt / (n * n) : 0};
+              double xi[3] = {B->xq[q[0]], B->xq[q[1]], dim == 3 ? B->xq[q[2]] : 0.};
+              double x[3], J[3][3];
+              multilinear_map<dim>(X, xi, x, J);
+              a[pt] = analytic_solution<dim>(P.sol_kind, P.sol_param, t,
+                                             P.vel_const, x, c);
+            }
+          __syncthreads();
+          // S^-1 in every direction (transform_from_q_points_to_basis)
+          sweep(&B->Sinv[0][0], max_n, a, b, 0, n, n, ext);
+          __syncthreads();
+          sweep(&B->Sinv[0][0], max_n, b, a, 1, n, n, ext);
+          __syncthreads();
+          double *res = a;
+          if (dim == 3)
+            {
+              sweep(&B->Sinv[0][0], max_n, a, b, 2, n, n, ext);
+              __syncthreads();
+              res = b;
+            }
+          for (int pt = threadIdx.x; pt < nd; pt += blockDim.x)
+            {
+              const int64_t k = (cell * nc + c) * nd + pt;
+              if (fp32)
+                static_cast<float *>(dst)[k] = (float)res[pt];
+              else
+                static_cast<double *>(dst)[k] = res[pt];
+            }
+          __syncthreads();
+        }
+    }
+
+    __device__ void
+    block_sum(double *red, double v, double *out)
+    {
+      red[threadIdx.x] = v;
+      __syncthreads();
+      for (int s = blockDim.x / 2; s > 0; s >>= 1)
+        {
+          if ((int)threadIdx.x < s)
+            red[threadIdx.x] += red[threadIdx.x + s];
+          __syncthreads();
+        }
+      if (threadIdx.x == 0)
+        *out = red[0];
+      __syncthreads();
+    }
+
+    // per cell: mass, energy, H1 seminorm (advection_solver.cc:787-818)
+    template <int dim>
+    __global__ void
+    mass_energy_kernel(const OpParams P, const DevBasis<double> *__restrict__ B,
+                       int n, const void *vec, int fp32, double *partials)
+    {
+      __shared__ double a[max_pts], b[max_pts];
+      __shared__ double X[24];
+      __shared__ double red[aux_threads];
+      constexpr int     NV   = 1 << dim;
+      const int64_t     cell = blockIdx.x;
+      const int         nd   = dim == 3 ? n * n * n : n * n;
+      int               ext[3] = {n, n, dim == 3 ? n : 1};
+      for (int e = threadIdx.x; e < NV * dim; e += blockDim.x)
+        X[e] = P.cell_vertices[cell * NV * dim + e];
+      for (int pt = threadIdx.x; pt < nd; pt += blockDim.x)
+        a[pt] = fp32 ? (double)load_as<float>(vec, cell * nd + pt) :
+                       load_as<double>(vec, cell * nd + pt);
+      __syncthreads();
+      sweep(&B->S[0][0], max_n, a, b, 0, n, n, ext);
+      __syncthreads();
+      sweep(&B->S[0][0], max_n, b, a, 1, n, n, ext);
+      __syncthreads();
+      double *uq = a;
+      if (dim == 3)
+        {
+          sweep(&B->S[0][0], max_n, a, b, 2, n, n, ext);
+          __syncthreads();
+          uq = b;
+        }
+      double m = 0, en = 0, h1 = 0;
+      for (int pt = threadIdx.x; pt < nd; pt += blockDim.x)
+        {
+          const int q[3] = {pt % n, (pt / n) % n, dim == 3 ? pt / (n * n) : 0};
+          double    xi[3] = {B->xq[q[0]], B->xq[q[1]], dim == 3 ? B->xq[q[2]] : 0.};
+          double    x[3], J[3][3], adj[3][3];
+          multilinear_map<dim>(X, xi, x, J);
+          const double det = adjugate<dim>(J, adj);
+          double       w   = B->w[q[0]] * B->w[q[1]] * (dim == 3 ? B->w[q[2]] : 1.);
+          const double JxW = det * w;
+          double       gref[3] = {0, 0, 0};
+          for (int d = 0; d < dim; ++d)
+            {
+              int qq[3] = {q[0], q[1], q[2]};
+              for (int p = 0; p < n; ++p)
+                {
+                  qq[d] = p;
+                  gref[d] += B->D[q[d]][p] * uq[qq[0] + n * (qq[1] + n * qq[2])];
+                }
+            }
+          double g2 = 0;
+          for (int e = 0; e < dim; ++e)
+            {
+              double ge = 0;
+              for (int d = 0; d < dim; ++d)
+                ge += adj[d][e] * gref[d];
+              ge /= det;
+              g2 += ge * ge;
+            }
+          const double u = uq[pt];
+          m += u * JxW;
+          en += u * u * JxW;
+          h1 += g2 * JxW;
+        }
+      block_sum(red, m, &partials[cell * 3 + 0]);
+      block_sum(red, en, &partials[cell * 3 + 1]);
+      block_sum(red, h1, &partials[cell * 3 + 2]);
+    }
+
+    // per cell: squared L2 error with the (n+1)-point rule and squared
+    // norm of the analytic solution with the n-point rule
+    // (advection_solver.cc:1187-1208)
+    template <int dim>
+    __global__ void
+    l2_error_kernel(const OpParams P, const DevBasis<double> *__restrict__ B,
+                    const QuadTables Q, int n, const void *vec, int fp32,
+                    double t, double *partials)
+    {
+      __shared__ double a[max_pts], b[max_pts];
+      __shared__ double X[24];
+      __shared__ double red[aux_threads];
+      constexpr int     NV   = 1 << dim;
+      const int64_t     cell = blockIdx.x;
+      const int         nd   = dim == 3 ? n * n * n : n * n;
+      const int         n2   = Q.n2;
+      for (int e = threadIdx.x; e < NV * dim; e += blockDim.x)
+        X[e] = P.cell_vertices[cell * NV * dim + e];
+      for (int pt = threadIdx.x; pt < nd; pt += blockDim.x)
+        a[pt] = fp32 ? (double)load_as<float>(vec, cell * nd + pt) :
+                       load_as<double>(vec, cell * nd + pt);
+      __syncthreads();
+      int ext[3] = {n, n, dim == 3 ? n : 1};
+      sweep(&Q.S2[0][0], max_n, a, b, 0, n2, n, ext);
+      ext[0] = n2;
+      __syncthreads();
+      sweep(&Q.S2[0][0], max_n, b, a, 1, n2, n, ext);
+      ext[1] = n2;
+      __syncthreads();
+      double *uq = a;
+      if (dim == 3)
+        {
+          sweep(&Q.S2[0][0], max_n, a, b, 2, n2, n, ext);
+          __syncthreads();
+          uq = b;
+        }
+      const int nd2 = dim == 3 ? n2 * n2 * n2 : n2 * n2;
+      double    err = 0, nrm = 0;
+      for (int pt = threadIdx.x; pt < nd2; pt += blockDim.x)
+        {
+          const int q[3] = {pt % n2, (pt / n2) % n2, dim == 3 ? pt / (n2 * n2) : 0};
+          double    xi[3] = {Q.x2[q[0]], Q.x2[q[1]], dim == 3 ? Q.x2[q[2]] : 0.};
+          double    x[3], J[3][3], adj[3][3];
+          multilinear_map<dim>(X, xi, x, J);
+          const double det = adjugate<dim>(J, adj);
+          const double w = Q.w2[q[0]] * Q.w2[q[1]] * (dim == 3 ? Q.w2[q[2]] : 1.);
+          const double g = analytic_solution<dim>(P.sol_kind, P.sol_param, t,
+                                                  P.vel_const, x);
+          err += (uq[pt] - g) * (uq[pt] - g) * det * w;
+        }
+      for (int pt = threadIdx.x; pt < nd; pt += blockDim.x)
+        {
+          const int q[3] = {pt % n, (pt / n) % n, dim == 3 ? pt / (n * n) : 0};
+          double    xi[3] = {B->xq[q[0]], B->xq[q[1]], dim == 3 ? B->xq[q[2]] : 0.};
+          double    x[3], J[3][3], adj[3][3];
+          multilinear_map<dim>(X, xi, x, J);
+          const double det = adjugate<dim>(J, adj);
+          const double w = B->w[q[0]] * B->w[q[1]] * (dim == 3 ? B->w[q[2]] : 1.);
+          const double g = analytic_solution<dim>(P.sol_kind, P.sol_param, t,
+                                                  P.vel_const, x);
+          nrm += g * g * det * w;
+        }
+      block_sum(red, err, &partials[cell * 2 + 0]);
+      block_sum(red, nrm, &partials[cell * 2 + 1]);
+    }
+
+    template <int dim>
+    __global__ void
+    fdm_cell_data_kernel(const OpParams P,
+                         const DevBasis<double> *__restrict__ B, int n,
+                         double *velocity, double *invdet)
+    {
+      __shared__ double X[24];
+      __shared__ double red[aux_threads];
+      constexpr int     NV   = 1 << dim;
+      const int64_t     cell = blockIdx.x;
+      const int         nd   = dim == 3 ? n * n * n : n * n;
+      for (int e = threadIdx.x; e < NV * dim; e += blockDim.x)
+        X[e] = P.cell_vertices[cell * NV * dim + e];
+      __syncthreads();
+      double va[3] = {0, 0, 0}, vol = 0;
+      for (int pt = threadIdx.x; pt < nd; pt += blockDim.x)
+        {
+          const int q[3] = {pt % n, (pt / n) % n, dim == 3 ? pt / (n * n) : 0};
+          double    xi[3] = {B->xq[q[0]], B->xq[q[1]], dim == 3 ? B->xq[q[2]] : 0.};
+          double    x[3], J[3][3], adj[3][3], a[3] = {0, 0, 0};
+          multilinear_map<dim>(X, xi, x, J);
+          const double det = adjugate<dim>(J, adj);
+          velocity_field<dim>(P.vel_kind, P.vel_const, x, a);
+          const double w =
+            B->w[q[0]] * B->w[q[1]] * (dim == 3 ? B->w[q[2]] : 1.);
+          // (J^-1 a) JxW = adj(J) a w
+          for (int d = 0; d < dim; ++d)
+            {
+              double s = 0;
+              for (int e = 0; e < dim; ++e)
+                s += adj[d][e] * a[e];
+              va[d] += s * w;
+            }
+          vol += det * w;
+          if (pt == 0)
+            invdet[cell] = 1. / det;
+        }
+      double tot[3], v;
+      for (int d = 0; d < dim; ++d)
+        block_sum(red, va[d], &tot[d]);
+      block_sum(red, vol, &v);
+      if (threadIdx.x == 0)
+        for (int d = 0; d < dim; ++d)
+          velocity[cell * dim + d] = tot[d] / v;
+    }
+
+    template <int dim, typename T>
+    __global__ void
+    inverse_mass_kernel(const OpParams P, const DevBasis<double> *__restrict__ B,
+                        int n, T *vec)
+    {
+      __shared__ double a[max_pts], b[max_pts];
+      __shared__ double X[24];
+      constexpr int     NV   = 1 << dim;
+      const int64_t     cell = blockIdx.x;
+      const int         nd   = dim == 3 ? n * n * n : n * n;
+      int               ext[3] = {n, n, dim == 3 ? n : 1};
+      // transposed inverse shape matrix for the S^-T sweeps
+      __shared__ double SinvT[max_n * max_n];
+      for (int e = threadIdx.x; e < n * n; e += blockDim.x)
+        SinvT[(e % n) * max_n + e / n] = B->Sinv[e / n][e % n];
+      for (int e = threadIdx.x; e < NV * dim; e += blockDim.x)
+        X[e] = P.cell_vertices[cell * NV * dim + e];
+      for (int pt = threadIdx.x; pt < nd; pt += blockDim.x)
+        a[pt] = (double)vec[cell * nd + pt];
+      __syncthreads();
+      sweep(SinvT, max_n, a, b, 0, n, n, ext);
+      __syncthreads();
+      sweep(SinvT, max_n, b, a, 1, n, n, ext);
+      __syncthreads();
+      double *q = a, *o = b;
+      if (dim == 3)
+        {
+          sweep(SinvT, max_n, a, b, 2, n, n, ext);
+          __syncthreads();
+          q = b;
+          o = a;
+        }
+      for (int pt = threadIdx.x; pt < nd; pt += blockDim.x)
+        {
+          const int qi[3] = {pt % n, (pt / n) % n, dim == 3 ? pt / (n * n) : 0};
+          double xi[3] = {B->xq[qi[0]], B->xq[qi[1]], dim == 3 ? B->xq[qi[2]] : 0.};
+          double x[3], J[3][3], adj[3][3];
+          multilinear_map<dim>(X, xi, x, J);
+          const double det = adjugate<dim>(J, adj);
+          const double w =
+            B->w[qi[0]] * B->w[qi[1]] * (dim == 3 ? B->w[qi[2]] : 1.);
+          q[pt] /= det * w;
+        }
+      __syncthreads();
+      sweep(&B->Sinv[0][0], max_n, q, o, 0, n, n, ext);
+      __syncthreads();
+      sweep(&B->Sinv[0][0], max_n, o, q, 1, n, n, ext);
+      __syncthreads();
+      double *res = q;
+      if (dim == 3)
+        {
+          sweep(&B->Sinv[0][0], max_n, q, o, 2, n, n, ext);
+          __syncthreads();
+          res = o;
+        }
+      for (int pt = threadIdx.x; pt < nd; pt += blockDim.x)
+        vec[cell * nd + pt] = (T)res[pt];
+    }
+
+    // sum of `k` interleaved columns of partials[count][k]
+    __global__ void
+    reduce_columns(const double *partials, int64_t count, int k, double *out)
+    {
+      __shared__ double red[1024];
+      for (int c = 0; c < k; ++c)
+        {
+          double s = 0;
+          for (int64_t i = threadIdx.x; i < count; i += blockDim.x)
+            s += partials[i * k + c];
+          red[threadIdx.x] = s;
+          __syncthreads();
+          for (int st = blockDim.x / 2; st > 0; st >>= 1)
+            {
+              if ((int)threadIdx.x < st)
+                red[threadIdx.x] += red[threadIdx.x + st];
+              __syncthreads();
+            }
+          if (threadIdx.x == 0)
+            out[c] = red[0];
+          __syncthreads();
+        }
+    }
+
+    // ---- vector kernels ----------------------------------------------------
+
+    constexpr int vec_threads = 256;
+    constexpr int max_multi   = 16;
+
+    struct Coeffs
+    {
+      double v[max_multi];
+    };
+
+    template <typename T>
+    __global__ void
+    axpby_kernel(int64_t n, T a, const T *__restrict__ x, T b, T *y)
+    {
+      for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+           i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = (b == (T)0) ? a * x[i] : fma(a, x[i], b * y[i]);
+    }
+
+    template <typename T, int M>
+    __global__ void
+    multi_axpy_kernel(int64_t n, const Coeffs c, const T *__restrict__ x,
+                      int64_t xs, T *y)
+    {
+      for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+           i += (int64_t)gridDim.x * blockDim.x)
+        {
+          T s = y[i];
+#pragma unroll
+          for (int j = 0; j < M; ++j)
+            s = fma((T)c.v[j], x[j * xs + i], s);
+          y[i] = s;
+        }
+    }
+
+    template <typename T, int M>
+    __global__ void
+    multi_dot_kernel(int64_t n, const T *__restrict__ x, int64_t xs,
+                     const T *__restrict__ y, double *partials)
+    {
+      __shared__ double red[M][vec_threads / 64];
+      double            s[M];
+#pragma unroll
+      for (int j = 0; j < M; ++j)
+        s[j] = 0;
+      for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+           i += (int64_t)gridDim.x * blockDim.x)
+        {
+          const double yi = (double)y[i];
+#pragma unroll
+          for (int j = 0; j < M; ++j)
+            s[j] = fma((double)x[j * xs + i], yi, s[j]);
+        }
+      const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+#pragma unroll
+      for (int j = 0; j < M; ++j)
+        {
+          double v = s[j];
+          for (int off = 32; off > 0; off >>= 1)
+            v += __shfl_down(v, off, 64);
+          if (lane == 0)
+            red[j][wave] = v;
+        }
+      __syncthreads();
+      if (threadIdx.x < M)
+        {
+          double v = 0;
+          for (int w = 0; w < vec_threads / 64; ++w)
+            v += red[threadIdx.x][w];
+          partials[blockIdx.x * M + threadIdx.x] = v;
+        }
+    }
+
+    template <typename TS, typename TD>
+    __global__ void
+    convert_kernel(int64_t n, const TS *__restrict__ x, TD *__restrict__ y)
+    {
+      for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+           i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = (TD)x[i];
+    }
+
+    template <typename T>
+    __global__ void
+    lsrk_update_kernel(int64_t n, T b, T a, const T *k, const T *sol_in,
+                       T *sol_out, T *ri_out)
+    {
+      for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+           i += (int64_t)gridDim.x * blockDim.x)
+        {
+          const T ki = k[i];
+          const T si = sol_in[i];
+          sol_out[i] = fma(b, ki, si);
+          if (ri_out)
+            ri_out[i] = fma(a, ki, si);
+        }
+    }
+
+    unsigned
+    grid_for(int64_t n)
+    {
+      const int64_t b = (n + vec_threads - 1) / vec_threads;
+      return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, 4096));
+    }
+
+    template <typename T>
+    void
+    multi_dot_t(Operator &op, int64_t n, int m, const T *x, int64_t xs,
+                const T *y, double *result)
+    {
+      const unsigned nb = std::min<unsigned>(grid_for(n), 1024);
+      op.d_partials.alloc(sizeof(double) * nb * max_multi);
+      op.d_result.alloc(sizeof(double) * 64);
+      double *part = op.d_partials.as<double>();
+      int     done = 0;
+      std::vector<double> host(m);
+      while (done < m)
+        {
+          const int chunk = std::min(max_multi, m - done);
+          const T  *xc    = x + done * xs;
+#define DG_MD(M)                                                              \
+  case M:                                                                     \
+    hipLaunchKernelGGL((multi_dot_kernel<T, M>), dim3(nb), dim3(vec_threads), \
+                       0, op.stream, n, xc, xs, y, part);                     \
+    break;
+          switch (chunk)
+            {
+              DG_MD(1) DG_MD(2) DG_MD(3) DG_MD(4) DG_MD(5) DG_MD(6) DG_MD(7)
+              DG_MD(8) DG_MD(9) DG_MD(10) DG_MD(11) DG_MD(12) DG_MD(13)
+              DG_MD(14) DG_MD(15) DG_MD(16)
+            }
+#undef DG_MD
+          hip_check(hipGetLastError(), "multi_dot");
+          hipLaunchKernelGGL(reduce_columns, dim3(1), dim3(1024), 0, op.stream,
+                             part, (int64_t)nb, chunk,
+                             op.d_result.as<double>() + done);
+          hip_check(hipGetLastError(), "reduce");
+          done += chunk;
+        }
+      hip_check(hipMemcpyAsync(host.data(), op.d_result.ptr,
+                               sizeof(double) * m, hipMemcpyDeviceToHost,
+                               op.stream),
+                "copy result");
+      hip_check(hipStreamSynchronize(op.stream), "sync");
+      for (int j = 0; j < m; ++j)
+        result[j] = host[j];
+    }
+
+    template <typename T>
+    void
+    multi_axpy_t(Operator &op, int64_t n, int m, const double *alpha,
+                 const T *x, int64_t xs, T *y)
+    {
+      int done = 0;
+      while (done < m)
+        {
+          const int chunk = std::min(max_multi, m - done);
+          Coeffs    c     = {};
+          for (int j = 0; j < chunk; ++j)
+            c.v[j] = alpha[done + j];
+          const T *xc = x + done * xs;
+#define DG_MA(M)                                                             \
+  case M:                                                                    \
+    hipLaunchKernelGGL((multi_axpy_kernel<T, M>), dim3(grid_for(n)),         \
+                       dim3(vec_threads), 0, op.stream, n, c, xc, xs, y);    \
+    break;
+          switch (chunk)
+            {
+              DG_MA(1) DG_MA(2) DG_MA(3) DG_MA(4) DG_MA(5) DG_MA(6) DG_MA(7)
+              DG_MA(8) DG_MA(9) DG_MA(10) DG_MA(11) DG_MA(12) DG_MA(13)
+              DG_MA(14) DG_MA(15) DG_MA(16)
+            }
+#undef DG_MA
+          hip_check(hipGetLastError(), "multi_axpy");
+          done += chunk;
+        }
+    }
+
+    QuadTables
+    make_quad_tables(int n)
+    {
+      QuadTables Q = {};
+      Q.n2         = n + 1;
+      std::vector<double> gp, gw;
+      gauss_points(n + 1, gp, gw);
+      const auto xl = gauss_lobatto_points_ext(n);
+      for (int q = 0; q < n + 1; ++q)
+        {
+          Q.w2[q] = gw[q];
+          Q.x2[q] = gp[q];
+          for (int i = 0; i < n; ++i)
+            {
+              real_ext v, d;
+              lagrange(xl, i, (real_ext)gp[q], v, d);
+              Q.S2[q][i] = (double)v;
+            }
+        }
+      return Q;
+    }
+
+    void
+    reduce_to_host(Operator &op, int64_t count, int k, double *out)
+    {
+      op.d_result.alloc(sizeof(double) * 64);
+      hipLaunchKernelGGL(reduce_columns, dim3(1), dim3(1024), 0, op.stream,
+                         op.d_partials.as<double>(), count, k,
+                         op.d_result.as<double>());
+      hip_check(hipGetLastError(), "reduce");
+      hip_check(hipMemcpyAsync(out, op.d_result.ptr, sizeof(double) * k,
+                               hipMemcpyDeviceToHost, op.stream),
+                "copy result");
+      hip_check(hipStreamSynchronize(op.stream), "sync");
+    }
+  } // namespace
+
+  void
+  aux_project(const Operator &op, void *dst, double t)
+  {
+    const OpParams P = op.base_params(t, 1);
+    const int      nc = op.cfg.n_components;
+    if (op.mesh.n_cells == 0)
+      return;
+    if (op.cfg.dim == 3)
+      hipLaunchKernelGGL(project_kernel<3>, dim3(op.mesh.n_cells),
+                         dim3(aux_threads), 0, op.stream, P,
+                         op.d_basis_d.as<DevBasis<double>>(), op.n, nc, t,
+                         dst, (int)op.fp32());
+    else
+      hipLaunchKernelGGL(project_kernel<2>, dim3(op.mesh.n_cells),
+                         dim3(aux_threads), 0, op.stream, P,
+                         op.d_basis_d.as<DevBasis<double>>(), op.n, nc, t,
+                         dst, (int)op.fp32());
+    hip_check(hipGetLastError(), "project");
+  }
+
+  void
+  aux_inverse_mass(const Operator &op, int p, void *vec)
+  {
+    const OpParams P = op.base_params(0, 1);
+    if (op.mesh.n_cells == 0)
+      return;
+    const auto *B = op.d_basis_d.as<DevBasis<double>>();
+    const dim3  g(op.mesh.n_cells), b(aux_threads);
+    if (op.cfg.dim == 3 && p == DG_FP64)
+      hipLaunchKernelGGL((inverse_mass_kernel<3, double>), g, b, 0, op.stream,
+                         P, B, op.n, (double *)vec);
+    else if (op.cfg.dim == 3)
+      hipLaunchKernelGGL((inverse_mass_kernel<3, float>), g, b, 0, op.stream,
+                         P, B, op.n, (float *)vec);
+    else if (p == DG_FP64)
+      hipLaunchKernelGGL((inverse_mass_kernel<2, double>), g, b, 0, op.stream,
+                         P, B, op.n, (double *)vec);
+    else
+      hipLaunchKernelGGL((inverse_mass_kernel<2, float>), g, b, 0, op.stream,
+                         P, B, op.n, (float *)vec);
+    hip_check(hipGetLastError(), "inverse mass");
+  }
+
+  void
+  aux_fdm_cell_data(const Operator &op, double *velocity, double *invdet)
+  {
+    const OpParams P = op.base_params(0, 1);
+    if (op.mesh.n_cells == 0)
+      return;
+    if (op.cfg.dim == 3)
+      hipLaunchKernelGGL(fdm_cell_data_kernel<3>, dim3(op.mesh.n_cells),
+                         dim3(aux_threads), 0, op.stream, P,
+                         op.d_basis_d.as<DevBasis<double>>(), op.n, velocity,
+                         invdet);
+    else
+      hipLaunchKernelGGL(fdm_cell_data_kernel<2>, dim3(op.mesh.n_cells),
+                         dim3(aux_threads), 0, op.stream, P,
+                         op.d_basis_d.as<DevBasis<double>>(), op.n, velocity,
+                         invdet);
+    hip_check(hipGetLastError(), "fdm cell data");
+  }
+
+  void
+  aux_mass_energy(const Operator &op_c, const void *vec, double out[3])
+  {
+    Operator &op = const_cast<Operator &>(op_c);
+    op.d_partials.alloc(sizeof(double) * 3 * std::max<int64_t>(1, op.mesh.n_cells));
+    const OpParams P = op.base_params(0, 1);
+    if (op.cfg.dim == 3)
+      hipLaunchKernelGGL(mass_energy_kernel<3>, dim3(op.mesh.n_cells),
+                         dim3(aux_threads), 0, op.stream, P,
+                         op.d_basis_d.as<DevBasis<double>>(), op.n, vec,
+                         (int)op.fp32(), op.d_partials.as<double>());
+    else
+      hipLaunchKernelGGL(mass_energy_kernel<2>, dim3(op.mesh.n_cells),
+                         dim3(aux_threads), 0, op.stream, P,
+                         op.d_basis_d.as<DevBasis<double>>(), op.n, vec,
+                         (int)op.fp32(), op.d_partials.as<double>());
+    hip_check(hipGetLastError(), "mass_energy");
+    reduce_to_host(op, op.mesh.n_cells, 3, out);
+  }
+
+  void
+  aux_l2_error(const Operator &op_c, const void *vec, double t, double out[2])
+  {
+    Operator &op = const_cast<Operator &>(op_c);
+    op.d_partials.alloc(sizeof(double) * 2 * std::max<int64_t>(1, op.mesh.n_cells));
+    const OpParams   P = op.base_params(t, 1);
+    const QuadTables Q = make_quad_tables(op.n);
+    if (op.cfg.dim == 3)
+      hipLaunchKernelGGL(l2_error_kernel<3>, dim3(op.mesh.n_cells),
+                         dim3(aux_threads), 0, op.stream, P,
+                         op.d_basis_d.as<DevBasis<double>>(), Q, op.n, vec,
+                         (int)op.fp32(), t, op.d_partials.as<double>());
+    else
+      hipLaunchKernelGGL(l2_error_kernel<2>, dim3(op.mesh.n_cells),
+                         dim3(aux_threads), 0, op.stream, P,
+                         op.d_basis_d.as<DevBasis<double>>(), Q, op.n, vec,
+                         (int)op.fp32(), t, op.d_partials.as<double>());
+    hip_check(hipGetLastError(), "l2_error");
+    double s[2];
+    reduce_to_host(op, op.mesh.n_cells, 2, s);
+    out[0] = std::sqrt(s[0]);
+    out[1] = std::sqrt(s[1]);
+  }
+
+  void
+  vec_axpby(Operator &op, int p, int64_t n, double a, const void *x, double b,
+            void *y)
+  {
+    if (n <= 0)
+      return;
+    if (p == DG_FP32)
+      hipLaunchKernelGGL(axpby_kernel<float>, dim3(grid_for(n)),
+                         dim3(vec_threads), 0, op.stream, n, (float)a,
+                         (const float *)x, (float)b, (float *)y);
+    else
+      hipLaunchKernelGGL(axpby_kernel<double>, dim3(grid_for(n)),
+                         dim3(vec_threads), 0, op.stream, n, a,
+                         (const double *)x, b, (double *)y);
+    hip_check(hipGetLastError(), "axpby");
+  }
+
+  void
+  vec_multi_axpy(Operator &op, int p, int64_t n, int m, const double *alpha,
+                 const void *x, int64_t xs, void *y)
+  {
+    if (n <= 0 || m <= 0)
+      return;
+    if (p == DG_FP32)
+      multi_axpy_t<float>(op, n, m, alpha, (const float *)x, xs, (float *)y);
+    else
+      multi_axpy_t<double>(op, n, m, alpha, (const double *)x, xs,
+                           (double *)y);
+  }
+
+  void
+  vec_multi_dot(Operator &op, int p, int64_t n, int m, const void *x,
+                int64_t xs, const void *y, double *result)
+  {
+    if (m <= 0)
+      return;
+    if (n <= 0)
+      {
+        for (int j = 0; j < m; ++j)
+          result[j] = 0;
+        return;
+      }
+    if (p == DG_FP32)
+      multi_dot_t<float>(op, n, m, (const float *)x, xs, (const float *)y,
+                         result);
+    else
+      multi_dot_t<double>(op, n, m, (const double *)x, xs, (const double *)y,
+                          result);
+  }
+
+  void
+  vec_copy_convert(Operator &op, int64_t n, int ps, const void *src, int pd,
+                   void *dst)
+  {
+    if (n <= 0)
+      return;
+    const dim3 g(grid_for(n)), b(vec_threads);
+    if (ps == DG_FP64 && pd == DG_FP64)
+      hipLaunchKernelGGL((convert_kernel<double, double>), g, b, 0, op.stream,
+                         n, (const double *)src, (double *)dst);
+    else if (ps == DG_FP64 && pd == DG_FP32)
+      hipLaunchKernelGGL((convert_kernel<double, float>), g, b, 0, op.stream,
+                         n, (const double *)src, (float *)dst);
+    else if (ps == DG_FP32 && pd == DG_FP64)
+      hipLaunchKernelGGL((convert_kernel<float, double>), g, b, 0, op.stream,
+                         n, (const float *)src, (double *)dst);
+    else
+      hipLaunchKernelGGL((convert_kernel<float, float>), g, b, 0, op.stream,
+                         n, (const float *)src, (float *)dst);
+    hip_check(hipGetLastError(), "convert");
+  }
+
+  void
+  vec_lsrk_update(Operator &op, int p, int64_t n, double b, double a,
+                  const void *k, const void *sol_in, void *sol_out,
+                  void *ri_out)
+  {
+    if (n <= 0)
+      return;
+    if (p == DG_FP32)
+      hipLaunchKernelGGL(lsrk_update_kernel<float>, dim3(grid_for(n)),
+                         dim3(vec_threads), 0, op.stream, n, (float)b,
+                         (float)a, (const float *)k, (const float *)sol_in,
+                         (float *)sol_out, (float *)ri_out);
+    else
+      hipLaunchKernelGGL(lsrk_update_kernel<double>, dim3(grid_for(n)),
+                         dim3(vec_threads), 0, op.stream, n, b, a,
+                         (const double *)k, (const double *)sol_in,
+                         (double *)sol_out, (double *)ri_out);
+    hip_check(hipGetLastError(), "lsrk_update");
+  }
+} // namespace dgops

@@ -1,0 +1,51 @@
+// Device copy of the 1D tables of one degree, in the arithmetic type of a
+// kernel. Kernels receive a pointer to it and read it with uniform
+// addresses, so the values come through the scalar cache.
+#pragma once
+
+#include "basis.hpp"
+
+namespace dgops
+{
+  template <typename T>
+  struct DevBasis
+  {
+    T S[max_n][max_n];    // nodal basis at Gauss points [q][i]
+    T Sinv[max_n][max_n]; // inverse of S [i][q]
+    T D[max_n][max_n];    // collocation derivative [q][p]
+    T G[max_n][max_n];    // nodal basis derivative at Gauss points [q][i]
+    T l0[max_n], l1[max_n];     // Gauss-Lagrange values at 0 and 1
+    T dl0[max_n], dl1[max_n];   // Gauss-Lagrange derivatives at 0 and 1
+    T dphi0[max_n], dphi1[max_n];
+    T w[max_n];  // Gauss weights on [0,1]
+    T xq[max_n]; // Gauss points on [0,1]
+    T xn[max_n]; // Gauss-Lobatto nodes on [0,1]
+  };
+
+  template <typename T>
+  inline DevBasis<T>
+  make_dev_basis(const Basis1D &b)
+  {
+    DevBasis<T> d = {};
+    for (int i = 0; i < max_n; ++i)
+      {
+        for (int j = 0; j < max_n; ++j)
+          {
+            d.S[i][j]    = (T)b.S[i][j];
+            d.Sinv[i][j] = (T)b.Sinv[i][j];
+            d.D[i][j]    = (T)b.D[i][j];
+            d.G[i][j]    = (T)b.G[i][j];
+          }
+        d.l0[i]    = (T)b.l0[i];
+        d.l1[i]    = (T)b.l1[i];
+        d.dl0[i]   = (T)b.dl0[i];
+        d.dl1[i]   = (T)b.dl1[i];
+        d.dphi0[i] = (T)b.dphi0[i];
+        d.dphi1[i] = (T)b.dphi1[i];
+        d.w[i]     = (T)b.gauss_wts[i];
+        d.xq[i]    = (T)b.gauss_pts[i];
+        d.xn[i]    = (T)b.gl_pts[i];
+      }
+    return d;
+  }
+} // namespace dgops

@@ -1,0 +1,663 @@
+// C ABI of the library: handle management, the explicit operator and the
+// low-storage Runge-Kutta integrator, index export, vector operations.
+#include <cmath>
+#include <cstring>
+#include <map>
+
+#include "aux.hpp"
+#include "internal.hpp"
+#include "registry.hpp"
+
+namespace dgops
+{
+  std::map<uint32_t, OpLaunch> &
+  op_registry()
+  {
+    static std::map<uint32_t, OpLaunch> m;
+    return m;
+  }
+
+  namespace
+  {
+    thread_local std::string last_error;
+
+    template <typename F>
+    int
+    guard(F &&f)
+    {
+      try
+        {
+          f();
+          return DG_OK;
+        }
+      catch (const Error &e)
+        {
+          last_error = e.what();
+          return e.code;
+        }
+      catch (const std::exception &e)
+        {
+          last_error = e.what();
+          return DG_ERR_ARGUMENT;
+        }
+      catch (...)
+        {
+          last_error = "unknown error";
+          return DG_ERR_ARGUMENT;
+        }
+    }
+
+    void
+    require(bool cond, const std::string &msg, int code = DG_ERR_ARGUMENT)
+    {
+      if (!cond)
+        throw Error(code, msg);
+    }
+
+    Operator *
+    get(dg_operator *op)
+    {
+      require(op != nullptr, "null operator handle");
+      return reinterpret_cast<Operator *>(op);
+    }
+    const Operator *
+    get(const dg_operator *op)
+    {
+      require(op != nullptr, "null operator handle");
+      return reinterpret_cast<const Operator *>(op);
+    }
+
+    // Low-storage Runge-Kutta coefficients (b_i, a_i) of the schemes of
+    // advection_solver.cc:828-897 (Kennedy, Carpenter, Lewis 2000;
+    // Tselios, Simos 2007)
+    void
+    lsrk_coefficients(int scheme, std::vector<double> &bi,
+                      std::vector<double> &ai)
+    {
+      switch (scheme)
+        {
+          case DG_LSRK_3_3:
+            bi = {0.245170287303492, 0.184896052186740, 0.569933660509768};
+            ai = {0.755726351946097, 0.386954477304099};
+            break;
+          case DG_LSRK_5_4:
+            bi = {1153189308089. / 22510343858157.,
+                  1772645290293. / 4653164025191.,
+                  -1672844663538. / 4480602732383.,
+                  2114624349019. / 3568978502595.,
+                  5198255086312. / 14908931495163.};
+            ai = {970286171893. / 4311952581923.,
+                  6584761158862. / 12103376702013.,
+                  2251764453980. / 15575788980749.,
+                  26877169314380. / 34165994151039.};
+            break;
+          case DG_LSRK_7_4:
+            bi = {0.0941840925477795334, 0.149683694803496998,
+                  0.285204742060440058,  -0.122201846148053668,
+                  0.0605151571191401122, 0.345986987898399296,
+                  0.186627171718797670};
+            ai = {0.241566650129646868 + bi[0], 0.0423866513027719953 + bi[1],
+                  0.215602732678803776 + bi[2], 0.232328007537583987 + bi[3],
+                  0.256223412574146438 + bi[4], 0.0978694102142697230 + bi[5]};
+            break;
+          case DG_LSRK_9_5:
+            bi = {2274579626619. / 23610510767302.,
+                  693987741272. / 12394497460941.,
+                  -347131529483. / 15096185902911.,
+                  1144057200723. / 32081666971178.,
+                  1562491064753. / 11797114684756.,
+                  13113619727965. / 44346030145118.,
+                  393957816125. / 7825732611452.,
+                  720647959663. / 6565743875477.,
+                  3559252274877. / 14424734981077.};
+            ai = {1107026461565. / 5417078080134.,
+                  38141181049399. / 41724347789894.,
+                  493273079041. / 11940823631197.,
+                  1851571280403. / 6147804934346.,
+                  11782306865191. / 62590030070788.,
+                  9452544825720. / 13648368537481.,
+                  4435885630781. / 26285702406235.,
+                  2357909744247. / 11371140753790.};
+            break;
+          default:
+            throw Error(DG_ERR_ARGUMENT, "unknown LSRK scheme");
+        }
+    }
+  } // namespace
+
+  void
+  set_last_error(const std::string &msg)
+  {
+    last_error = msg;
+  }
+
+  double
+  velocity_time_factor(const dg_config &cfg, double t)
+  {
+    if (cfg.time_factor == DG_TIME_FACTOR_COS)
+      return std::cos(pi * t / cfg.final_time);
+    return 1.;
+  }
+
+  OpParams
+  Operator::base_params(double t, int n_comp) const
+  {
+    OpParams P = {};
+    P.n_cells       = mesh.n_cells;
+    P.neighbors     = d_neighbors.as<int>();
+    P.cell_vertices = d_vertices.as<double>();
+    for (int d = 0; d < 3; ++d)
+      {
+        P.h[d]         = mesh.h[d];
+        P.vel_const[d] = cfg.velocity[d];
+      }
+    P.cartesian = cfg.perturbation == 0.;
+    P.vel_kind  = cfg.velocity_kind;
+    const double tf = velocity_time_factor(cfg, t);
+    for (int c = 0; c < 8; ++c)
+      P.vel_factor[c] = c < n_comp ? tf : 0.;
+    P.alpha    = cfg.flux_alpha;
+    P.skew     = cfg.factor_skew;
+    P.sign     = -1.;
+    P.has_mass = 0;
+    P.sol_kind = cfg.solution_kind;
+    P.sol_time = t;
+    for (int i = 0; i < 8; ++i)
+      P.sol_param[i] = cfg.solution_param[i];
+    P.viscosity = cfg.viscosity;
+    P.penalty   = cfg.penalty_factor;
+    return P;
+  }
+
+  void *
+  Operator::solver_vector(size_t idx, size_t bytes)
+  {
+    while (solver_vectors.size() <= idx)
+      solver_vectors.emplace_back(new DevBuf());
+    solver_vectors[idx]->alloc(bytes);
+    return solver_vectors[idx]->ptr;
+  }
+
+  void
+  launch_operator(const Operator &op,
+                  const OpParams &P,
+                  int             nc,
+                  bool            fp32,
+                  bool            skew,
+                  int             fin,
+                  int             epi,
+                  const void     *io)
+  {
+    const uint32_t key = op_key(op.cfg.dim, op.n, nc, fp32 ? 1 : 0,
+                                P.cartesian ? 1 : 0, skew ? 1 : 0, fin, epi);
+    auto it = op_registry().find(key);
+    if (it == op_registry().end())
+      throw Error(DG_ERR_UNSUPPORTED,
+                  "operator variant not built: dim=" +
+                    std::to_string(op.cfg.dim) +
+                    " degree=" + std::to_string(op.n - 1) +
+                    " components=" + std::to_string(nc) +
+                    (fp32 ? " fp32" : " fp64") +
+                    (P.cartesian ? " cartesian" : " multilinear") +
+                    (skew ? " skew" : "") + " fin=" + std::to_string(fin) +
+                    " epi=" + std::to_string(epi));
+    hip_check(it->second(P, op.basis_ptr(fp32), op.d_basis_d.ptr, io,
+                         op.stream),
+              "operator launch");
+  }
+
+  namespace
+  {
+    void
+    validate(const dg_config &c)
+    {
+      require(c.dim == 2 || c.dim == 3, "dim must be 2 or 3");
+      require(c.degree >= 1 && c.degree <= max_n - 1, "degree must be 1..7",
+              DG_ERR_UNSUPPORTED);
+      require(c.precision == DG_FP64 || c.precision == DG_FP32,
+              "bad precision");
+      require(c.operator_kind >= 0 && c.operator_kind <= 2,
+              "bad operator kind");
+      if (c.operator_kind == DG_OP_BURGERS)
+        require(c.n_components == c.dim, "Burgers needs dim components");
+      else
+        require(c.n_components == 1, "scalar operator needs 1 component");
+      require(c.refine >= 0 && c.refine <= 12, "refine out of range");
+      for (int d = 0; d < c.dim; ++d)
+        require(c.n_coarse[d] >= 1 && c.hi[d] > c.lo[d], "bad mesh box");
+      require(c.perturbation >= 0 && c.perturbation < 0.5,
+              "perturbation must be in [0, 0.5)");
+      require(c.time_factor == DG_TIME_FACTOR_ONE || c.final_time > 0,
+              "final_time must be positive for the cos time factor");
+      require(c.irk_stages >= 0 && c.irk_stages <= 5, "irk_stages 0..5");
+      if (c.velocity_kind == DG_VEL_CELLULAR_3D)
+        require(c.dim == 3, "cellular velocity field is 3D");
+    }
+  } // namespace
+} // namespace dgops
+
+using namespace dgops;
+
+extern "C" {
+
+const char *
+dg_last_error(void)
+{
+  return last_error.c_str();
+}
+
+void
+dg_config_default(dg_config *c)
+{
+  if (!c)
+    return;
+  std::memset(c, 0, sizeof(*c));
+  c->dim           = 3;
+  c->degree        = 4;
+  c->n_components  = 1;
+  c->operator_kind = DG_OP_ADVECTION;
+  c->precision     = DG_FP64;
+  for (int d = 0; d < 3; ++d)
+    {
+      c->n_coarse[d] = 1;
+      c->lo[d]       = 0.;
+      c->hi[d]       = 1.;
+      c->periodic[d] = 1;
+    }
+  c->refine         = 2;
+  c->perturbation   = 0.;
+  c->seed           = 42;
+  c->velocity_kind  = DG_VEL_CONSTANT;
+  c->velocity[0]    = 1.;
+  c->time_factor    = DG_TIME_FACTOR_ONE;
+  c->final_time     = 1.;
+  c->flux_alpha     = 1.;
+  c->factor_skew    = 0.;
+  c->viscosity      = 0.;
+  c->penalty_factor = 0.;
+  c->solution_kind  = DG_SOL_ZERO;
+  c->irk_stages     = 3;
+}
+
+int
+dg_create(const dg_config *cfg, void *stream, dg_operator **out)
+{
+  return guard([&] {
+    require(cfg && out, "null argument");
+    *out = nullptr;
+    validate(*cfg);
+    auto op    = std::make_unique<Operator>();
+    op->cfg    = *cfg;
+    op->stream = (hipStream_t)stream;
+    hip_check(hipGetDevice(&op->device), "hipGetDevice");
+    MeshDesc md = {};
+    md.dim      = cfg->dim;
+    for (int d = 0; d < 3; ++d)
+      {
+        md.n_coarse[d] = d < cfg->dim ? cfg->n_coarse[d] : 1;
+        md.lo[d]       = cfg->lo[d];
+        md.hi[d]       = cfg->hi[d];
+        md.periodic[d] = cfg->periodic[d];
+      }
+    md.refine       = cfg->refine;
+    md.perturbation = cfg->perturbation;
+    md.seed         = cfg->seed;
+    op->mesh        = make_mesh(md);
+    op->n           = cfg->degree + 1;
+    op->nd          = cfg->dim == 3 ? op->n * op->n * op->n : op->n * op->n;
+    op->n_dofs      = op->mesh.n_cells * op->nd;
+    op->basis       = make_basis(op->n);
+
+    const auto bd = make_dev_basis<double>(op->basis);
+    const auto bf = make_dev_basis<float>(op->basis);
+    op->d_basis_d.alloc(sizeof(bd));
+    op->d_basis_f.alloc(sizeof(bf));
+    hip_check(hipMemcpy(op->d_basis_d.ptr, &bd, sizeof(bd),
+                        hipMemcpyHostToDevice),
+              "upload basis");
+    hip_check(hipMemcpy(op->d_basis_f.ptr, &bf, sizeof(bf),
+                        hipMemcpyHostToDevice),
+              "upload basis");
+    const size_t nb = op->mesh.neighbors.size() * sizeof(int);
+    const size_t nv = op->mesh.cell_vertices.size() * sizeof(double);
+    op->d_neighbors.alloc(nb);
+    op->d_vertices.alloc(nv);
+    hip_check(hipMemcpy(op->d_neighbors.ptr, op->mesh.neighbors.data(), nb,
+                        hipMemcpyHostToDevice),
+              "upload neighbors");
+    hip_check(hipMemcpy(op->d_vertices.ptr, op->mesh.cell_vertices.data(), nv,
+                        hipMemcpyHostToDevice),
+              "upload vertices");
+    *out = reinterpret_cast<dg_operator *>(op.release());
+  });
+}
+
+int
+dg_destroy(dg_operator *op)
+{
+  return guard([&] { delete reinterpret_cast<Operator *>(op); });
+}
+
+int
+dg_set_stream(dg_operator *h, void *stream)
+{
+  return guard([&] { get(h)->stream = (hipStream_t)stream; });
+}
+
+int
+dg_get_info(const dg_operator *h, dg_info *info)
+{
+  return guard([&] {
+    const Operator *op = get(h);
+    require(info != nullptr, "null info");
+    info->n_cells       = op->mesh.n_cells;
+    info->n_dofs        = op->n_dofs;
+    info->vector_size   = op->n_dofs * op->cfg.n_components;
+    info->dofs_per_cell = op->nd;
+    for (int d = 0; d < 3; ++d)
+      {
+        info->cells_per_direction[d] = op->mesh.N[d];
+        info->h[d]                   = op->mesh.h[d];
+      }
+    info->min_vertex_distance = op->mesh.min_vertex_distance;
+  });
+}
+
+int
+dg_apply(dg_operator *h, const void *src, void *dst, double t)
+{
+  return guard([&] {
+    Operator *op = get(h);
+    require(op->cfg.operator_kind == DG_OP_ADVECTION,
+            "dg_apply: advection operator only (see dg_burgers/sipg)",
+            DG_ERR_STATE);
+    require(src && dst, "null vector");
+    require(src != dst, "dg_apply: src and dst must differ");
+    require(op->cfg.factor_skew == 0.,
+            "explicit operator is built for factor_skew = 0",
+            DG_ERR_UNSUPPORTED);
+    OpParams P = op->base_params(t, 1);
+    if (op->fp32())
+      {
+        OpBuffers<float> io = {};
+        io.src   = (const float *)src;
+        io.dst   = (float *)dst;
+        io.scale = 1.f;
+        launch_operator(*op, P, 1, true, false, fin_inverse_mass, epi_store,
+                        &io);
+      }
+    else
+      {
+        OpBuffers<double> io = {};
+        io.src   = (const double *)src;
+        io.dst   = (double *)dst;
+        io.scale = 1.;
+        launch_operator(*op, P, 1, false, false, fin_inverse_mass, epi_store,
+                        &io);
+      }
+  });
+}
+
+namespace dgops
+{
+  // fused stage: reads `ri_in` (all of it, including neighbours) and
+  // `sol_in` (own entries); writes sol_out and ri_out. ri_in must not alias
+  // an output.
+  static void
+  fused_stage(Operator &op, double t, double b_dt, double a_dt,
+              const void *ri_in, const void *sol_in, void *sol_out,
+              void *ri_out, bool last)
+  {
+    require(op.cfg.factor_skew == 0.,
+            "explicit operator is built for factor_skew = 0",
+            DG_ERR_UNSUPPORTED);
+    OpParams P = op.base_params(t, 1);
+    if (op.fp32())
+      {
+        OpBuffers<float> io = {};
+        io.src     = (const float *)ri_in;
+        io.sol_in  = (const float *)sol_in;
+        io.sol_out = (float *)sol_out;
+        io.ri_out  = (float *)ri_out;
+        io.b_dt    = (float)b_dt;
+        io.a_dt    = (float)a_dt;
+        io.last    = last;
+        launch_operator(op, P, 1, true, false, fin_inverse_mass, epi_lsrk,
+                        &io);
+      }
+    else
+      {
+        OpBuffers<double> io = {};
+        io.src     = (const double *)ri_in;
+        io.sol_in  = (const double *)sol_in;
+        io.sol_out = (double *)sol_out;
+        io.ri_out  = (double *)ri_out;
+        io.b_dt    = b_dt;
+        io.a_dt    = a_dt;
+        io.last    = last;
+        launch_operator(op, P, 1, false, false, fin_inverse_mass, epi_lsrk,
+                        &io);
+      }
+  }
+} // namespace dgops
+
+int
+dg_perform_stage(dg_operator *h, double t, double b_dt, double a_dt,
+                 const void *current_ri, void *vec_ki, void *solution,
+                 void *next_ri)
+{
+  return guard([&] {
+    Operator *op = get(h);
+    require(op->cfg.operator_kind == DG_OP_ADVECTION,
+            "perform_stage: advection operator only", DG_ERR_STATE);
+    require(current_ri && vec_ki && solution && next_ri, "null vector");
+    const bool last = a_dt == 0.;
+    if (current_ri != solution && current_ri != next_ri)
+      {
+        fused_stage(*op, t, b_dt, a_dt, current_ri, solution, solution,
+                    last ? nullptr : next_ri, last);
+        return;
+      }
+    // aliased registers as in LowStorageRungeKuttaIntegrator: two passes
+    // through vec_ki like the reference (loop, then inverse mass + update)
+    require(vec_ki != current_ri,
+            "perform_stage: vec_ki must not alias current_ri");
+    require(dg_apply(h, current_ri, vec_ki, t) == DG_OK, last_error);
+    vec_lsrk_update(*op, op->cfg.precision, op->n_dofs, b_dt, a_dt, vec_ki,
+                    solution, solution, last ? nullptr : next_ri);
+  });
+}
+
+int
+dg_lsrk_n_stages(int scheme)
+{
+  std::vector<double> b, a;
+  try
+    {
+      lsrk_coefficients(scheme, b, a);
+    }
+  catch (...)
+    {
+      return -1;
+    }
+  return (int)b.size();
+}
+
+int
+dg_lsrk_time_step(dg_operator *h, int scheme, double t, double dt,
+                  void *solution, void *vec_ri, void *vec_ki)
+{
+  return guard([&] {
+    Operator *op = get(h);
+    require(op->cfg.operator_kind == DG_OP_ADVECTION,
+            "lsrk_time_step: advection operator only", DG_ERR_STATE);
+    require(solution && vec_ri && vec_ki, "null vector");
+    require(solution != vec_ri && solution != vec_ki && vec_ri != vec_ki,
+            "the three registers must be distinct");
+    std::vector<double> bi, ai;
+    lsrk_coefficients(scheme, bi, ai);
+    const int ns = (int)bi.size();
+    // Three registers rotate so that the stage input is never written by
+    // the fused kernel that reads it; the last stage writes `solution`.
+    void *buf[3] = {solution, vec_ri, vec_ki};
+    int   r_in = 0, s_in = 0;
+    double sum_previous_bi = 0;
+    for (int stage = 0; stage < ns; ++stage)
+      {
+        const double c_i =
+          stage == 0 ? 0. : sum_previous_bi + ai[stage - 1];
+        if (stage > 0)
+          sum_previous_bi += bi[stage - 1];
+        const double tt   = t + c_i * dt;
+        const bool   last = stage == ns - 1;
+        if (last)
+          {
+            fused_stage(*op, tt, bi[stage] * dt, 0., buf[r_in], buf[s_in],
+                        buf[0], nullptr, true);
+            break;
+          }
+        int cand[2], k = 0;
+        for (int b = 0; b < 3; ++b)
+          if (b != r_in)
+            cand[k++] = b;
+        int s_out, r_out;
+        if (stage == ns - 2)
+          {
+            // the next (last) stage reads r_out and writes buffer 0
+            r_out = cand[0] == 0 ? cand[1] : cand[0];
+            s_out = cand[0] == r_out ? cand[1] : cand[0];
+          }
+        else if (cand[0] == s_in || cand[1] == s_in)
+          {
+            s_out = s_in;
+            r_out = cand[0] == s_in ? cand[1] : cand[0];
+          }
+        else
+          {
+            s_out = cand[0];
+            r_out = cand[1];
+          }
+        fused_stage(*op, tt, bi[stage] * dt, ai[stage] * dt, buf[r_in],
+                    buf[s_in], buf[s_out], buf[r_out], false);
+        r_in = r_out;
+        s_in = s_out;
+      }
+  });
+}
+
+int
+dg_project_initial(dg_operator *h, void *dst, double t)
+{
+  return guard([&] {
+    Operator *op = get(h);
+    require(dst != nullptr, "null vector");
+    aux_project(*op, dst, t);
+  });
+}
+
+int
+dg_compute_mass_and_energy(dg_operator *h, const void *vec, double out[3])
+{
+  return guard([&] {
+    Operator *op = get(h);
+    require(vec && out, "null argument");
+    require(op->cfg.n_components == 1, "scalar fields only");
+    aux_mass_energy(*op, vec, out);
+  });
+}
+
+int
+dg_l2_error(dg_operator *h, const void *vec, double t, double out[2])
+{
+  return guard([&] {
+    Operator *op = get(h);
+    require(vec && out, "null argument");
+    require(op->cfg.n_components == 1, "scalar fields only");
+    aux_l2_error(*op, vec, t, out);
+  });
+}
+
+int
+dg_vector_dot(dg_operator *h, int p, int64_t n, const void *x, const void *y,
+              double *result)
+{
+  return guard([&] {
+    require(x && y && result, "null argument");
+    vec_multi_dot(*get(h), p, n, 1, x, n, y, result);
+  });
+}
+
+int
+dg_vector_multi_dot(dg_operator *h, int p, int64_t n, int m, const void *x,
+                    const void *y, double *result)
+{
+  return guard([&] {
+    require(x && y && result && m >= 0, "bad argument");
+    vec_multi_dot(*get(h), p, n, m, x, n, y, result);
+  });
+}
+
+int
+dg_vector_multi_axpy(dg_operator *h, int p, int64_t n, int m,
+                     const double *alpha, const void *x, void *y)
+{
+  return guard([&] {
+    require(x && y && alpha && m >= 0, "bad argument");
+    vec_multi_axpy(*get(h), p, n, m, alpha, x, n, y);
+  });
+}
+
+int
+dg_vector_axpby(dg_operator *h, int p, int64_t n, double a, const void *x,
+                double b, void *y)
+{
+  return guard([&] {
+    require(x && y, "null vector");
+    vec_axpby(*get(h), p, n, a, x, b, y);
+  });
+}
+
+int
+dg_vector_copy_convert(dg_operator *h, int64_t n, int ps, const void *src,
+                       int pd, void *dst)
+{
+  return guard([&] {
+    require(src && dst, "null vector");
+    vec_copy_convert(*get(h), n, ps, src, pd, dst);
+  });
+}
+
+int
+dg_export_cell_coordinates(const dg_operator *h, int32_t *ijk)
+{
+  return guard([&] {
+    const Operator *op = get(h);
+    require(ijk != nullptr, "null output");
+    for (size_t i = 0; i < op->mesh.cell_ijk.size(); ++i)
+      ijk[i] = op->mesh.cell_ijk[i];
+  });
+}
+
+int
+dg_export_neighbors(const dg_operator *h, int32_t *nb)
+{
+  return guard([&] {
+    const Operator *op = get(h);
+    require(nb != nullptr, "null output");
+    for (size_t i = 0; i < op->mesh.neighbors.size(); ++i)
+      nb[i] = op->mesh.neighbors[i];
+  });
+}
+
+int
+dg_export_cell_vertices(const dg_operator *h, double *v)
+{
+  return guard([&] {
+    const Operator *op = get(h);
+    require(v != nullptr, "null output");
+    std::memcpy(v, op->mesh.cell_vertices.data(),
+                op->mesh.cell_vertices.size() * sizeof(double));
+  });
+}
+
+} // extern "C"

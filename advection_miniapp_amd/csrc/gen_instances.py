@@ -1,0 +1,96 @@
+"""Writes the translation units that instantiate the kernel variants.
+
+Run by build.py before compiling; each generated file registers the
+launchers it contains in the kernel table (registry.hpp).
+"""
+import itertools
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT = os.path.join(HERE, "generated")
+
+# (dim, n, nc, fp32, cart, skew, fin, epi)
+FIN_INVMASS, FIN_INTEGRATE = 0, 1
+EPI_STORE, EPI_LSRK = 0, 1
+
+
+def op_variants():
+    v = []
+    # explicit right-hand side M^-1(-L u): apply and fused LSRK stage
+    for dim, n, cart, epi in itertools.product(
+            (2, 3), range(2, 9), (0, 1), (EPI_STORE, EPI_LSRK)):
+        v.append((dim, n, 1, 0, cart, 0, FIN_INVMASS, epi))
+    # implicit single-stage operator (M/dt + L) and IRK right-hand side,
+    # skew-symmetric form, fp64 and fp32
+    for dim, n, cart, fp32 in itertools.product(
+            (2, 3), range(2, 9), (0, 1), (0, 1)):
+        v.append((dim, n, 1, fp32, cart, 1, FIN_INTEGRATE, EPI_STORE))
+    # stage-coupled IRK operator
+    for dim, n, nc, fp32 in itertools.product(
+            (2, 3), range(2, 8), (2, 3, 4), (0, 1)):
+        v.append((dim, n, nc, fp32, 1, 1, FIN_INTEGRATE, EPI_STORE))
+        if dim == 3 and n == 5:
+            v.append((dim, n, nc, fp32, 0, 1, FIN_INTEGRATE, EPI_STORE))
+    return v
+
+
+def fdm_variants():
+    v = []
+    for dim, n, ns, prec in itertools.product((2, 3), range(2, 9), (1, 2, 3, 4),
+                                              (0, 1, 2)):
+        v.append(("fdm", dim, n, ns, prec))
+    return v
+
+
+FDM_TYPES = {0: ("double", "double", "double"), 1: ("float", "float", "float"),
+             2: ("float", "double", "double")}
+
+
+def emit(path, variants):
+    lines = ['#include "../registry.hpp"', "", "namespace dgops", "{"]
+    for i, var in enumerate(variants):
+        if var[0] == "fdm":
+            _, dim, n, ns, prec = var
+            tc, ti, to = FDM_TYPES[prec]
+            lines.append(
+                f"  static FdmRegistrar f{i}(fdm_key({dim}, {n}, {ns}, {prec}), "
+                f"&launch_fdm<{dim}, {n}, {ns}, {tc}, {ti}, {to}>);")
+            continue
+        dim, n, nc, fp32, cart, skew, fin, epi = var
+        T = "float" if fp32 else "double"
+        lines.append(
+            f"  static OpRegistrar r{i}(op_key({dim}, {n}, {nc}, {fp32}, {cart}, "
+            f"{skew}, {fin}, {epi}), &launch_op<{dim}, {n}, {nc}, {T}, "
+            f"{'true' if cart else 'false'}, {'true' if skew else 'false'}, "
+            f"{fin}, {epi}>);")
+    lines += ["} // namespace dgops", ""]
+    text = "\n".join(lines)
+    if os.path.exists(path) and open(path).read() == text:
+        return
+    with open(path, "w") as f:
+        f.write(text)
+
+
+def main(n_files=16):
+    os.makedirs(OUT, exist_ok=True)
+    def weight(v):
+        if v[0] == "fdm":
+            return v[2] ** (v[1] + 1) * v[3]
+        return v[1] ** (v[0] + 1) * v[2]
+    variants = op_variants() + fdm_variants()
+    # spread heavy (3D, large n) variants evenly
+    variants.sort(key=lambda v: -weight(v))
+    groups = [[] for _ in range(n_files)]
+    for i, v in enumerate(variants):
+        groups[i % n_files].append(v)
+    names = []
+    for g, vs in enumerate(groups):
+        name = os.path.join(OUT, f"op_{g:02d}.hip")
+        emit(name, vs)
+        names.append(name)
+    return names
+
+
+if __name__ == "__main__":
+    for n in main():
+        print(n)

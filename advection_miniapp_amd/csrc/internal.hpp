@@ -1,0 +1,136 @@
+// Host-side state of an operator handle, shared by the host translation
+// units of the library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/dgops.h"
+#include "basis.hpp"
+#include "dev_basis.hpp"
+#include "dg_common.hpp"
+#include "mesh.hpp"
+
+namespace dgops
+{
+  struct Error : std::runtime_error
+  {
+    int code;
+    Error(int c, const std::string &m)
+      : std::runtime_error(m)
+      , code(c)
+    {}
+  };
+
+  inline void
+  hip_check(hipError_t e, const char *what)
+  {
+    if (e != hipSuccess)
+      throw Error(DG_ERR_HIP,
+                  std::string(what) + ": " + hipGetErrorString(e));
+  }
+
+  void set_last_error(const std::string &msg);
+
+  // device buffer owned by a handle
+  struct DevBuf
+  {
+    void  *ptr   = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+      if (ptr)
+        (void)hipFree(ptr);
+      ptr   = nullptr;
+      bytes = 0;
+    }
+    void alloc(size_t b)
+    {
+      if (b <= bytes && ptr)
+        return;
+      release();
+      hip_check(hipMalloc(&ptr, b ? b : 8), "hipMalloc");
+      bytes = b;
+    }
+    template <typename T>
+    T *as() const
+    {
+      return static_cast<T *>(ptr);
+    }
+  };
+
+  // FDM data of one cell-independent 1D problem (advection sign c)
+  struct FdmTables1D
+  {
+    // real-pair eigenbasis: D = M^-1 K, D V = V Lambda (block diagonal)
+    double V[max_n][max_n];     // [i][j]
+    double VinvMinv[max_n][max_n];
+    double lambda_re[max_n], lambda_im[max_n]; // per column of V; pairs
+    int    pair[max_n];         // -1: real, else index of partner column
+  };
+
+  struct IrkData
+  {
+    int                 s = 0;
+    std::vector<double> A, invA, b, c; // s*s, s*s, s, s
+    // real-pair eigen decomposition invA = T Lambda T^-1
+    std::vector<double> T, invT, lam_re, lam_im;
+    std::vector<int>    pair;
+  };
+
+  struct Operator
+  {
+    dg_config      cfg;
+    StructuredMesh mesh;
+    int            n;      // points per direction
+    int            nd;     // dofs per cell per component
+    int64_t        n_dofs; // per component
+    hipStream_t    stream = nullptr;
+    int            device = 0;
+
+    Basis1D basis;
+    DevBuf  d_basis_d, d_basis_f, d_neighbors, d_vertices;
+
+    // implicit state
+    double time      = 0;
+    double time_step = 0;
+    IrkData irk;
+    FdmTables1D fdm[2]; // velocity sign + and -
+    DevBuf  d_fdm_d, d_fdm_f, d_cell_velocity, d_cell_invdet;
+
+    // scratch for reductions and solvers
+    DevBuf d_partials, d_result;
+    std::vector<DevBuf *> scratch_vectors;
+    std::vector<std::unique_ptr<DevBuf>> solver_vectors;
+
+    OpParams base_params(double time, int n_comp) const;
+    const void *basis_ptr(bool fp32) const
+    {
+      return fp32 ? d_basis_f.ptr : d_basis_d.ptr;
+    }
+    bool fp32() const { return cfg.precision == DG_FP32; }
+    size_t scalar_size() const { return fp32() ? 4 : 8; }
+    void  *solver_vector(size_t idx, size_t bytes);
+  };
+
+  // kernels launched from host code
+  void launch_operator(const Operator &op,
+                       const OpParams &P,
+                       int             nc,
+                       bool            fp32,
+                       bool            skew,
+                       int             fin,
+                       int             epi,
+                       const void     *io);
+
+  double velocity_time_factor(const dg_config &cfg, double t);
+} // namespace dgops

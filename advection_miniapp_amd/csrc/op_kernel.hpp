@@ -1,0 +1,816 @@
+// Fused matrix-free DG advection operator for gfx950.
+//
+// One workgroup handles a batch of cells; inside a cell every thread owns
+// one 1D line of (k+1) points (n^(dim-1) threads per cell) and keeps it in
+// registers. Contractions along the owned line run in registers; changing
+// the line direction goes through LDS (write the line, barrier, read the
+// line of the new direction).
+//
+// The operator is evaluated in quadrature space: with S the nodal-to-Gauss
+// interpolation matrix, D the collocation derivative at the Gauss points
+// and l0/l1 the Gauss-Lagrange values at the faces,
+//   integrate(values v, gradients g, face values f) =
+//        S^T (v + sum_d D_d^T g_d + sum_faces l_face (x) f)
+// so the inverse mass matrix M^-1 = S^-1 diag(1/JxW) S^-T collapses to
+//   M^-1 integrate(...) = S^-1 diag(1/JxW) (v + sum_d D_d^T g_d + ...).
+// This is the same operator as deal.II's FEEvaluation::integrate followed
+// by CellwiseInverseMassMatrix::apply (advection_solver.cc:495-513 and
+// :635-657), computed with 9 one-dimensional sweeps per cell instead of
+// 12 + 6 in two passes.
+//
+// Faces are processed element-centric: every cell computes the numerical
+// flux on its 2*dim faces with its own outward normal, reading the
+// neighbour's face nodes (Gauss-Lobatto nodes make the trace a plain
+// subset of the neighbour's values). The flux for the plus side of a
+// reference face equals the minus-side flux with the normal reversed
+// (advection_solver.cc:560-575), so the result matches the face-centric
+// loop of the reference and every dof is written exactly once.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "analytic.hpp"
+#include "dev_basis.hpp"
+#include "dg_common.hpp"
+
+namespace dgops
+{
+  template <int dim, int n>
+  struct Lay
+  {
+    static constexpr int nl = dim == 3 ? n * n : n; // lines per cell
+    static constexpr int nd = nl * n;               // dofs per cell
+    static constexpr int stride = nd | 1;           // LDS stride per cell
+
+    // storage offset of point i on line p of direction dir
+    template <int dir>
+    __device__ static __forceinline__ int
+    off(int p, int i)
+    {
+      if constexpr (dim == 3)
+        {
+          if constexpr (dir == 0)
+            return i + n * p;
+          else if constexpr (dir == 1)
+            return (p % n) + n * i + n * n * (p / n);
+          else
+            return p + n * n * i;
+        }
+      else
+        {
+          if constexpr (dir == 0)
+            return i + n * p;
+          else
+            return p + n * i;
+        }
+    }
+
+    template <int dir>
+    __device__ static __forceinline__ void
+    idx(int p, int i, int (&q)[3])
+    {
+      if constexpr (dim == 3)
+        {
+          if constexpr (dir == 0)
+            {
+              q[0] = i;
+              q[1] = p % n;
+              q[2] = p / n;
+            }
+          else if constexpr (dir == 1)
+            {
+              q[0] = p % n;
+              q[1] = i;
+              q[2] = p / n;
+            }
+          else
+            {
+              q[0] = p % n;
+              q[1] = p / n;
+              q[2] = i;
+            }
+        }
+      else
+        {
+          if constexpr (dir == 0)
+            {
+              q[0] = i;
+              q[1] = p;
+            }
+          else
+            {
+              q[0] = p;
+              q[1] = i;
+            }
+          q[2] = 0;
+        }
+    }
+  };
+
+  template <int dim, int n>
+  constexpr int
+  cells_per_block()
+  {
+    constexpr int nl = Lay<dim, n>::nl;
+    const int     c  = 256 / nl;
+    return c < 1 ? 1 : c;
+  }
+
+  // ---- small register contractions ---------------------------------------
+
+  // out[q] = sum_i M[q][i] in[i]
+  template <int n, typename T>
+  __device__ __forceinline__ void
+  mat_apply(const T (&M)[max_n][max_n], const T (&in)[n], T (&out)[n])
+  {
+#pragma unroll
+    for (int q = 0; q < n; ++q)
+      {
+        T s = M[q][0] * in[0];
+#pragma unroll
+        for (int i = 1; i < n; ++i)
+          s = fma(M[q][i], in[i], s);
+        out[q] = s;
+      }
+  }
+
+  // out[i] = sum_q M[q][i] in[q]
+  template <int n, typename T>
+  __device__ __forceinline__ void
+  mat_apply_t(const T (&M)[max_n][max_n], const T (&in)[n], T (&out)[n])
+  {
+#pragma unroll
+    for (int i = 0; i < n; ++i)
+      {
+        T s = M[0][i] * in[0];
+#pragma unroll
+        for (int q = 1; q < n; ++q)
+          s = fma(M[q][i], in[q], s);
+        out[i] = s;
+      }
+  }
+
+  // out[i] += sum_q M[q][i] in[q]
+  template <int n, typename T>
+  __device__ __forceinline__ void
+  mat_apply_t_add(const T (&M)[max_n][max_n], const T (&in)[n], T (&out)[n])
+  {
+#pragma unroll
+    for (int i = 0; i < n; ++i)
+      {
+        T s = out[i];
+#pragma unroll
+        for (int q = 0; q < n; ++q)
+          s = fma(M[q][i], in[q], s);
+        out[i] = s;
+      }
+  }
+
+  template <int dim, int n, int dir, typename T>
+  __device__ __forceinline__ void
+  lds_write_line(T *buf, int p, const T (&v)[n])
+  {
+#pragma unroll
+    for (int i = 0; i < n; ++i)
+      buf[Lay<dim, n>::template off<dir>(p, i)] = v[i];
+  }
+
+  template <int dim, int n, int dir, typename T>
+  __device__ __forceinline__ void
+  lds_read_line(const T *buf, int p, T (&v)[n])
+  {
+#pragma unroll
+    for (int i = 0; i < n; ++i)
+      v[i] = buf[Lay<dim, n>::template off<dir>(p, i)];
+  }
+
+  // ---- geometry ------------------------------------------------------------
+
+  // Multilinear map of a cell from its 2^dim corners X[v*dim+e]:
+  // position x and Jacobian J[e][d] = dx_e / dxi_d at reference point xi.
+  template <int dim>
+  __device__ __forceinline__ void
+  multilinear_map(const double *X,
+                  const double (&xi)[3],
+                  double (&x)[3],
+                  double (&J)[3][3])
+  {
+    if constexpr (dim == 3)
+      {
+        const double a = xi[0], b = xi[1], c = xi[2];
+        const double ma = 1 - a, mb = 1 - b, mc = 1 - c;
+        for (int e = 0; e < 3; ++e)
+          {
+            const double x0 = X[0 * 3 + e], x1 = X[1 * 3 + e],
+                         x2 = X[2 * 3 + e], x3 = X[3 * 3 + e],
+                         x4 = X[4 * 3 + e], x5 = X[5 * 3 + e],
+                         x6 = X[6 * 3 + e], x7 = X[7 * 3 + e];
+            const double e01 = ma * x0 + a * x1, e23 = ma * x2 + a * x3,
+                         e45 = ma * x4 + a * x5, e67 = ma * x6 + a * x7;
+            const double f0 = mb * e01 + b * e23, f1 = mb * e45 + b * e67;
+            x[e] = mc * f0 + c * f1;
+            J[e][0] = mc * (mb * (x1 - x0) + b * (x3 - x2)) +
+                      c * (mb * (x5 - x4) + b * (x7 - x6));
+            J[e][1] = mc * (e23 - e01) + c * (e67 - e45);
+            J[e][2] = f1 - f0;
+          }
+      }
+    else
+      {
+        const double a = xi[0], b = xi[1];
+        const double ma = 1 - a, mb = 1 - b;
+        for (int e = 0; e < 2; ++e)
+          {
+            const double x0 = X[0 * 2 + e], x1 = X[1 * 2 + e],
+                         x2 = X[2 * 2 + e], x3 = X[3 * 2 + e];
+            const double e01 = ma * x0 + a * x1, e23 = ma * x2 + a * x3;
+            x[e]    = mb * e01 + b * e23;
+            J[e][0] = mb * (x1 - x0) + b * (x3 - x2);
+            J[e][1] = e23 - e01;
+          }
+        x[2] = 0;
+      }
+  }
+
+  // rows of the adjugate: adj[d][e] = det(J) * (J^-1)[d][e]
+  template <int dim>
+  __device__ __forceinline__ double
+  adjugate(const double (&J)[3][3], double (&adj)[3][3])
+  {
+    if constexpr (dim == 3)
+      {
+        // columns c_d = J[.][d]; adj row 0 = c1 x c2, row 1 = c2 x c0,
+        // row 2 = c0 x c1
+        for (int d = 0; d < 3; ++d)
+          {
+            const int d1 = (d + 1) % 3, d2 = (d + 2) % 3;
+            adj[d][0] = J[1][d1] * J[2][d2] - J[2][d1] * J[1][d2];
+            adj[d][1] = J[2][d1] * J[0][d2] - J[0][d1] * J[2][d2];
+            adj[d][2] = J[0][d1] * J[1][d2] - J[1][d1] * J[0][d2];
+          }
+        return J[0][0] * adj[0][0] + J[1][0] * adj[0][1] +
+               J[2][0] * adj[0][2];
+      }
+    else
+      {
+        adj[0][0] = J[1][1];
+        adj[0][1] = -J[0][1];
+        adj[1][0] = -J[1][0];
+        adj[1][1] = J[0][0];
+        return J[0][0] * J[1][1] - J[0][1] * J[1][0];
+      }
+  }
+
+  // Per quadrature point data of the cell term: cv[d] = w * (adj(J) a)_d
+  // (the reference-space transport speed times JxW, which is what
+  // advection_solver_implicit_real.cc:1126-1128 and :998-1007 fold
+  // together) and JxW = w det(J).
+  template <int dim, bool cartesian>
+  __device__ __forceinline__ void
+  cell_point_data(const OpParams &P,
+                  const double   *X,
+                  const double   *xq,
+                  const double   *wq,
+                  const int (&q)[3],
+                  double (&cv)[3],
+                  double &JxW)
+  {
+    double w = wq[q[0]] * wq[q[1]];
+    if (dim == 3)
+      w *= wq[q[2]];
+    double x[3], a[3] = {0, 0, 0};
+    if constexpr (cartesian)
+      {
+        for (int d = 0; d < dim; ++d)
+          x[d] = X[d] + P.h[d] * xq[q[d]];
+        velocity_field<dim>(P.vel_kind, P.vel_const, x, a);
+        double vol = P.h[0] * P.h[1];
+        if (dim == 3)
+          vol *= P.h[2];
+        for (int d = 0; d < dim; ++d)
+          cv[d] = w * (vol / P.h[d]) * a[d];
+        JxW = w * vol;
+      }
+    else
+      {
+        const double xi[3] = {xq[q[0]], xq[q[1]], dim == 3 ? xq[q[2]] : 0.};
+        double       J[3][3], adj[3][3];
+        multilinear_map<dim>(X, xi, x, J);
+        velocity_field<dim>(P.vel_kind, P.vel_const, x, a);
+        const double det = adjugate<dim>(J, adj);
+        for (int d = 0; d < dim; ++d)
+          {
+            double s = 0;
+            for (int e = 0; e < dim; ++e)
+              s += adj[d][e] * a[e];
+            cv[d] = w * s;
+          }
+        JxW = w * det;
+      }
+  }
+
+  // Face point data: an = a . n dA (outward normal of this cell, including
+  // the face quadrature weight) and the physical point x.
+  template <int dim, bool cartesian>
+  __device__ __forceinline__ void
+  face_point_data(const OpParams &P,
+                  const double   *X,
+                  const double   *xq,
+                  const double   *wq,
+                  int             dir,
+                  int             side,
+                  const int (&q)[3],
+                  double &an,
+                  double (&x)[3])
+  {
+    double w = 1.;
+    for (int e = 0; e < dim; ++e)
+      if (e != dir)
+        w *= wq[q[e]];
+    double xi[3];
+    for (int e = 0; e < 3; ++e)
+      xi[e] = (e < dim) ? xq[q[e]] : 0.;
+    xi[dir] = side ? 1. : 0.;
+    double a[3] = {0, 0, 0};
+    if constexpr (cartesian)
+      {
+        for (int d = 0; d < dim; ++d)
+          x[d] = X[d] + P.h[d] * xi[d];
+        x[2] = dim == 3 ? x[2] : 0.;
+        velocity_field<dim>(P.vel_kind, P.vel_const, x, a);
+        double area = 1.;
+        for (int e = 0; e < dim; ++e)
+          if (e != dir)
+            area *= P.h[e];
+        an = (side ? 1. : -1.) * w * area * a[dir];
+      }
+    else
+      {
+        double J[3][3], adj[3][3];
+        multilinear_map<dim>(X, xi, x, J);
+        velocity_field<dim>(P.vel_kind, P.vel_const, x, a);
+        adjugate<dim>(J, adj);
+        double s = 0;
+        for (int e = 0; e < dim; ++e)
+          s += adj[dir][e] * a[e];
+        an = (side ? 1. : -1.) * w * s;
+      }
+  }
+
+
+  // ---- face terms ------------------------------------------------------------
+
+  // Numerical flux on the two faces normal to `dir`, at the face point owned
+  // by this thread in the dir-line layout; adds sign * l_side (x) flux to the
+  // line accumulator. Positive-operator convention of
+  // advection_solver_implicit_real.cc:1032-1058:
+  //   F = 1/2 [(u- + u+) an + alpha |an| (u- - u+)] - skew an u-
+  // with an = a . n dA on this cell's outward normal.
+  template <int dim, int n, int dir, typename T, bool cartesian>
+  __device__ __forceinline__ void
+  face_terms(const OpParams &P,
+             const DevBasis<double> *Bd,
+             const DevBasis<T>      &B,
+             const double           *X,
+             int                     p,
+             bool                    active,
+             const int (&nb)[2 * dim],
+             const T (&uplus)[2 * dim],
+             const T (&uq)[n],
+             T (&acc)[n],
+             T    vf,
+             T (&fan)[2 * dim],
+             bool compute_geometry)
+  {
+    int q[3];
+    Lay<dim, n>::template idx<dir>(p, 0, q);
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+      {
+        const int f = 2 * dir + s;
+        const T(&l)[max_n] = s ? B.l1 : B.l0;
+        if (compute_geometry)
+          {
+            double an = 0, x[3];
+            if (active)
+              face_point_data<dim, cartesian>(P, X, Bd->xq, Bd->w, dir, s, q,
+                                              an, x);
+            fan[f] = (T)an;
+          }
+        T um = l[0] * uq[0];
+#pragma unroll
+        for (int i = 1; i < n; ++i)
+          um = fma(l[i], uq[i], um);
+        T up = uplus[f];
+        if (active && nb[f] < 0)
+          {
+            double an, x[3];
+            face_point_data<dim, cartesian>(P, X, Bd->xq, Bd->w, dir, s, q,
+                                            an, x);
+            up = (T)analytic_solution<dim>(P.sol_kind, P.sol_param,
+                                           P.sol_time, P.vel_const, x);
+          }
+        const T an = fan[f] * vf;
+        const T flux =
+          (T)0.5 * ((um + up) * an + (T)P.alpha * fabs(an) * (um - up)) -
+          (T)P.skew * an * um;
+        const T sf = (T)P.sign * flux;
+#pragma unroll
+        for (int i = 0; i < n; ++i)
+          acc[i] = fma(l[i], sf, acc[i]);
+      }
+  }
+
+  // ---- the kernel ------------------------------------------------------------
+
+  enum Finalize : int
+  {
+    fin_inverse_mass = 0, // S^-1 diag(1/JxW): explicit right-hand side
+    fin_integrate    = 1  // S^T: operator action (implicit vmult)
+  };
+
+  enum Epilogue : int
+  {
+    epi_store = 0, // dst = scale * y
+    epi_lsrk  = 1  // k = y; sol_out = sol_in + b k; ri_out = sol_in + a k
+  };
+
+  template <typename T>
+  struct OpBuffers
+  {
+    const T *src; // [NC][comp_stride]
+    T       *dst; // [NC][comp_stride]
+    int64_t  comp_stride;
+    T        scale;
+    // LSRK epilogue (NC == 1)
+    const T *sol_in;
+    T       *sol_out;
+    T       *ri_out;
+    T        b_dt, a_dt;
+    int      last;
+  };
+
+  // y_c = fin( sign * L_c u_c + sum_b mass[c][b] M u_b ), c < NC, with
+  // L_c the advection operator at velocity vel_factor[c] * a(x).
+  template <int dim, int n, int NC, typename T, bool cartesian, bool skew,
+            int fin, int epi>
+  __global__ void __launch_bounds__(256)
+  dg_operator_kernel(const OpParams                       P,
+                     const DevBasis<T> *__restrict__      Bp,
+                     const DevBasis<double> *__restrict__ Bd,
+                     const OpBuffers<T>                   io)
+  {
+    using L            = Lay<dim, n>;
+    constexpr int nl   = L::nl;
+    constexpr int nd   = L::nd;
+    constexpr int CPB  = cells_per_block<dim, n>();
+    constexpr int STR  = L::stride;
+    constexpr int FSTR = 2 * dim * nl;
+    constexpr int NV   = 1 << dim;
+    constexpr int LAST = dim - 1;
+    constexpr int NX   = cartesian ? dim : NV * dim;
+
+    __shared__ T      s_u[CPB * STR];
+    __shared__ T      s_acc[CPB * STR];
+    __shared__ T      s_g0[CPB * STR];
+    __shared__ T      s_g1[dim == 3 ? CPB * STR : 1];
+    __shared__ T      s_sk0[skew ? CPB * STR : 1];
+    __shared__ T      s_sk1[(skew && dim == 3) ? CPB * STR : 1];
+    __shared__ T      s_face[dim == 3 ? CPB * FSTR : 1];
+    __shared__ double s_X[CPB * NX];
+
+    const DevBasis<T> &B    = *Bp;
+    const int          tid  = threadIdx.x;
+    const int          lc   = tid / nl;
+    const int          p    = tid - lc * nl;
+    const int64_t      cell = (int64_t)blockIdx.x * CPB + lc;
+    const bool         active = (lc < CPB) && (cell < P.n_cells);
+    const int          lcs    = lc < CPB ? lc : 0;
+
+    T      *u_l   = s_u + lcs * STR;
+    T      *acc_l = s_acc + lcs * STR;
+    T      *g0_l  = s_g0 + lcs * STR;
+    T      *g1_l  = dim == 3 ? s_g1 + lcs * STR : s_g1;
+    T      *sk0_l = skew ? s_sk0 + lcs * STR : s_sk0;
+    T      *sk1_l = (skew && dim == 3) ? s_sk1 + lcs * STR : s_sk1;
+    T      *f_l   = dim == 3 ? s_face + lcs * FSTR : s_face;
+    double *X     = s_X + lcs * NX;
+
+    if (active)
+      for (int e = p; e < NX; e += nl)
+        X[e] = P.cell_vertices[cell * NV * dim + e];
+    int nb[2 * dim];
+#pragma unroll
+    for (int f = 0; f < 2 * dim; ++f)
+      nb[f] = active ? P.neighbors[cell * 2 * dim + f] : -1;
+    __syncthreads();
+
+    // ---- per component: neighbour traces and values at quadrature points
+    T uplus[NC][2 * dim];
+    T uqs[NC][n];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      {
+        const T *src = io.src + c * io.comp_stride;
+        if constexpr (dim == 3)
+          {
+            // step 1: thread (aq, j) interpolates the neighbour's face nodes
+            // along the first tangential direction
+            const int aq = p % n, j = p / n;
+#pragma unroll
+            for (int f = 0; f < 6; ++f)
+              {
+                const int d = f / 2, s = f % 2;
+                const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+                T         v  = 0;
+                if (active && nb[f] >= 0)
+                  {
+                    const T  *nsrc = src + (int64_t)nb[f] * nd;
+                    const int nn   = s ? 0 : n - 1;
+#pragma unroll
+                    for (int i = 0; i < n; ++i)
+                      {
+                        int ijk[3];
+                        ijk[d]  = nn;
+                        ijk[t1] = i;
+                        ijk[t2] = j;
+                        v = fma(B.S[aq][i],
+                                nsrc[ijk[0] + n * ijk[1] + n * n * ijk[2]], v);
+                      }
+                  }
+                f_l[f * nl + j * n + aq] = v;
+              }
+          }
+        // own values, sweep along x
+        T r[n], t[n];
+        if (active)
+          {
+            const T *csrc = src + cell * nd;
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+              r[i] = csrc[L::template off<0>(p, i)];
+          }
+        else
+          {
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+              r[i] = 0;
+          }
+        mat_apply<n>(B.S, r, t);
+        lds_write_line<dim, n, 0>(acc_l, p, t);
+        __syncthreads();
+        if constexpr (dim == 3)
+          {
+            // step 2 of the face interpolation: thread (aq, bq)
+            const int aq = p % n, bq = p / n;
+#pragma unroll
+            for (int f = 0; f < 6; ++f)
+              {
+                T v = 0;
+#pragma unroll
+                for (int jj = 0; jj < n; ++jj)
+                  v = fma(B.S[bq][jj], f_l[f * nl + jj * n + aq], v);
+                uplus[c][f] = v;
+              }
+          }
+        else
+          {
+#pragma unroll
+            for (int f = 0; f < 4; ++f)
+              {
+                const int d = f / 2, s = f % 2;
+                T         v = 0;
+                if (active && nb[f] >= 0)
+                  {
+                    const T  *nsrc = src + (int64_t)nb[f] * nd;
+                    const int nn   = s ? 0 : n - 1;
+#pragma unroll
+                    for (int i = 0; i < n; ++i)
+                      {
+                        const int ij0 = d == 0 ? nn : i;
+                        const int ij1 = d == 0 ? i : nn;
+                        v = fma(B.S[p][i], nsrc[ij0 + n * ij1], v);
+                      }
+                  }
+                uplus[c][f] = v;
+              }
+          }
+        lds_read_line<dim, n, 1>(acc_l, p, t);
+        mat_apply<n>(B.S, t, r);
+        if constexpr (dim == 3)
+          {
+            __syncthreads();
+            lds_write_line<dim, n, 1>(acc_l, p, r);
+            __syncthreads();
+            lds_read_line<dim, n, 2>(acc_l, p, t);
+            mat_apply<n>(B.S, t, r);
+          }
+#pragma unroll
+        for (int i = 0; i < n; ++i)
+          uqs[c][i] = r[i];
+        __syncthreads();
+      }
+
+    // ---- geometry and velocity at this thread's LAST-line points
+    T cv[3][n], jxw[n];
+#pragma unroll
+    for (int i = 0; i < n; ++i)
+      {
+        int    q[3];
+        double cvd[3] = {0, 0, 0}, J = 1.;
+        L::template idx<LAST>(p, i, q);
+        if (active)
+          cell_point_data<dim, cartesian>(P, X, Bd->xq, Bd->w, q, cvd, J);
+        cv[0][i] = (T)cvd[0];
+        cv[1][i] = (T)cvd[1];
+        cv[2][i] = (T)cvd[2];
+        jxw[i]   = (T)J;
+      }
+    T fan[2 * dim];
+
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+      {
+        const T vf = (T)P.vel_factor[c];
+        T       acc[n];
+        // ---- LAST direction (z in 3D, y in 2D) ----
+        {
+#pragma unroll
+          for (int i = 0; i < n; ++i)
+            acc[i] = 0;
+          if (P.has_mass)
+            {
+#pragma unroll
+              for (int b = 0; b < NC; ++b)
+                {
+                  const T m = (T)P.mass[c][b];
+#pragma unroll
+                  for (int i = 0; i < n; ++i)
+                    acc[i] = fma(m * jxw[i], uqs[b][i], acc[i]);
+                }
+            }
+          const T fcell = (T)(-(1. - P.skew) * P.sign) * vf;
+          T       g[n];
+#pragma unroll
+          for (int i = 0; i < n; ++i)
+            g[i] = fcell * cv[LAST][i] * uqs[c][i];
+          mat_apply_t_add<n>(B.D, g, acc);
+          if constexpr (skew)
+            {
+              T du[n];
+              mat_apply<n>(B.D, uqs[c], du);
+              const T fs = (T)(P.skew * P.sign) * vf;
+#pragma unroll
+              for (int i = 0; i < n; ++i)
+                acc[i] = fma(fs * cv[LAST][i], du[i], acc[i]);
+            }
+          face_terms<dim, n, LAST, T, cartesian>(P, Bd, B, X, p, active, nb,
+                                                 uplus[c], uqs[c], acc, vf,
+                                                 fan, c == 0);
+          lds_write_line<dim, n, LAST>(u_l, p, uqs[c]);
+          lds_write_line<dim, n, LAST>(acc_l, p, acc);
+#pragma unroll
+          for (int i = 0; i < n; ++i)
+            g[i] = fcell * cv[0][i] * uqs[c][i];
+          lds_write_line<dim, n, LAST>(g0_l, p, g);
+          if constexpr (dim == 3)
+            {
+#pragma unroll
+              for (int i = 0; i < n; ++i)
+                g[i] = fcell * cv[1][i] * uqs[c][i];
+              lds_write_line<dim, n, LAST>(g1_l, p, g);
+            }
+          if constexpr (skew)
+            {
+              const T fs = (T)(P.skew * P.sign) * vf;
+#pragma unroll
+              for (int i = 0; i < n; ++i)
+                g[i] = fs * cv[0][i];
+              lds_write_line<dim, n, LAST>(sk0_l, p, g);
+              if constexpr (dim == 3)
+                {
+#pragma unroll
+                  for (int i = 0; i < n; ++i)
+                    g[i] = fs * cv[1][i];
+                  lds_write_line<dim, n, LAST>(sk1_l, p, g);
+                }
+            }
+          __syncthreads();
+        }
+
+        // ---- y direction in 3D ----
+        if constexpr (dim == 3)
+          {
+            T uq[n], g[n];
+            lds_read_line<dim, n, 1>(u_l, p, uq);
+            lds_read_line<dim, n, 1>(g1_l, p, g);
+            lds_read_line<dim, n, 1>(acc_l, p, acc);
+            mat_apply_t_add<n>(B.D, g, acc);
+            if constexpr (skew)
+              {
+                T du[n], sk[n];
+                mat_apply<n>(B.D, uq, du);
+                lds_read_line<dim, n, 1>(sk1_l, p, sk);
+#pragma unroll
+                for (int i = 0; i < n; ++i)
+                  acc[i] = fma(sk[i], du[i], acc[i]);
+              }
+            face_terms<dim, n, 1, T, cartesian>(P, Bd, B, X, p, active, nb,
+                                                uplus[c], uq, acc, vf, fan,
+                                                c == 0);
+            lds_write_line<dim, n, 1>(acc_l, p, acc);
+            __syncthreads();
+          }
+
+        // ---- x direction ----
+        {
+          T uq[n], g[n];
+          lds_read_line<dim, n, 0>(u_l, p, uq);
+          lds_read_line<dim, n, 0>(g0_l, p, g);
+          lds_read_line<dim, n, 0>(acc_l, p, acc);
+          mat_apply_t_add<n>(B.D, g, acc);
+          if constexpr (skew)
+            {
+              T du[n], sk[n];
+              mat_apply<n>(B.D, uq, du);
+              lds_read_line<dim, n, 0>(sk0_l, p, sk);
+#pragma unroll
+              for (int i = 0; i < n; ++i)
+                acc[i] = fma(sk[i], du[i], acc[i]);
+            }
+          face_terms<dim, n, 0, T, cartesian>(P, Bd, B, X, p, active, nb,
+                                              uplus[c], uq, acc, vf, fan,
+                                              c == 0);
+        }
+        __syncthreads();
+        if constexpr (fin == fin_inverse_mass)
+          {
+            // 1/JxW sits at LAST-line points; move it to x-lines via LDS
+            T ij[n];
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+              ij[i] = (T)1 / jxw[i];
+            lds_write_line<dim, n, LAST>(g0_l, p, ij);
+            __syncthreads();
+            lds_read_line<dim, n, 0>(g0_l, p, ij);
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+              acc[i] *= ij[i];
+          }
+
+        // ---- back to nodal values: x (registers), y, (z) ----
+        {
+          T t[n];
+          if constexpr (fin == fin_inverse_mass)
+            mat_apply<n>(B.Sinv, acc, t);
+          else
+            mat_apply_t<n>(B.S, acc, t);
+          lds_write_line<dim, n, 0>(acc_l, p, t);
+          __syncthreads();
+          lds_read_line<dim, n, 1>(acc_l, p, t);
+          if constexpr (fin == fin_inverse_mass)
+            mat_apply<n>(B.Sinv, t, acc);
+          else
+            mat_apply_t<n>(B.S, t, acc);
+          if constexpr (dim == 3)
+            {
+              __syncthreads();
+              lds_write_line<dim, n, 1>(acc_l, p, acc);
+              __syncthreads();
+              lds_read_line<dim, n, 2>(acc_l, p, t);
+              if constexpr (fin == fin_inverse_mass)
+                mat_apply<n>(B.Sinv, t, acc);
+              else
+                mat_apply_t<n>(B.S, t, acc);
+            }
+        }
+
+        // ---- epilogue on LAST-direction lines (coalesced) ----
+        if (active)
+          {
+            const int64_t base = c * io.comp_stride + cell * nd;
+            if constexpr (epi == epi_store)
+              {
+#pragma unroll
+                for (int i = 0; i < n; ++i)
+                  io.dst[base + L::template off<LAST>(p, i)] =
+                    io.scale * acc[i];
+              }
+            else
+              {
+#pragma unroll
+                for (int i = 0; i < n; ++i)
+                  {
+                    const int64_t k  = base + L::template off<LAST>(p, i);
+                    const T       s0 = io.sol_in[k];
+                    io.sol_out[k]    = fma(io.b_dt, acc[i], s0);
+                    if (!io.last)
+                      io.ri_out[k] = fma(io.a_dt, acc[i], s0);
+                  }
+              }
+          }
+        if (NC > 1)
+          __syncthreads();
+      }
+  }
+} // namespace dgops

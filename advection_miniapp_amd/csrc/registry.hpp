@@ -1,0 +1,110 @@
+// Table of compiled kernel variants. Every generated translation unit
+// registers its launchers at load time; the host code looks them up by key
+// and reports DG_ERR_UNSUPPORTED for combinations that were not built.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+
+#include "fdm_kernel.hpp"
+#include "op_kernel.hpp"
+
+namespace dgops
+{
+  using OpLaunch = hipError_t (*)(const OpParams &,
+                                  const void *basis_t,
+                                  const void *basis_d,
+                                  const void *io,
+                                  hipStream_t);
+
+  inline uint32_t
+  op_key(int dim, int n, int nc, int fp32, int cart, int skew, int fin, int epi)
+  {
+    return (uint32_t)dim | (uint32_t)n << 2 | (uint32_t)nc << 6 |
+           (uint32_t)fp32 << 10 | (uint32_t)cart << 11 |
+           (uint32_t)skew << 12 | (uint32_t)fin << 13 | (uint32_t)epi << 15;
+  }
+
+  std::map<uint32_t, OpLaunch> &op_registry();
+
+  struct OpRegistrar
+  {
+    OpRegistrar(uint32_t key, OpLaunch f) { op_registry()[key] = f; }
+  };
+
+  // FDM preconditioner variants: prec 0 = fp64, 1 = fp32, 2 = fp32
+  // arithmetic on fp64 vectors
+  using FdmLaunch = hipError_t (*)(const FdmParams &,
+                                   const void *tables,
+                                   const void *src,
+                                   void       *dst,
+                                   hipStream_t);
+
+  inline uint32_t
+  fdm_key(int dim, int n, int ns, int prec)
+  {
+    return (uint32_t)dim | (uint32_t)n << 2 | (uint32_t)ns << 6 |
+           (uint32_t)prec << 10 | 1u << 20;
+  }
+
+  std::map<uint32_t, FdmLaunch> &fdm_registry();
+
+  struct FdmRegistrar
+  {
+    FdmRegistrar(uint32_t key, FdmLaunch f) { fdm_registry()[key] = f; }
+  };
+
+  template <int dim, int n, int NS, typename Tc, typename Tin, typename Tout>
+  hipError_t
+  launch_fdm(const FdmParams &P,
+             const void      *tables,
+             const void      *src,
+             void            *dst,
+             hipStream_t      stream)
+  {
+    constexpr int CPB    = cells_per_block<dim, n>();
+    constexpr int nl     = Lay<dim, n>::nl;
+    const int64_t blocks = (P.n_cells + CPB - 1) / CPB;
+    if (blocks <= 0)
+      return hipSuccess;
+    hipLaunchKernelGGL((fdm_kernel<dim, n, NS, Tc, Tin, Tout>),
+                       dim3((unsigned)blocks),
+                       dim3(CPB * nl),
+                       0,
+                       stream,
+                       P,
+                       (const FdmDev<Tc> *)tables,
+                       (const Tin *)src,
+                       (Tout *)dst);
+    return hipGetLastError();
+  }
+
+  template <int dim, int n, int NC, typename T, bool cart, bool skew, int fin,
+            int epi>
+  hipError_t
+  launch_op(const OpParams &P,
+            const void     *bt,
+            const void     *bd,
+            const void     *io,
+            hipStream_t     stream)
+  {
+    constexpr int CPB    = cells_per_block<dim, n>();
+    constexpr int nl     = Lay<dim, n>::nl;
+    const int64_t blocks = (P.n_cells + CPB - 1) / CPB;
+    if (blocks <= 0)
+      return hipSuccess;
+    hipLaunchKernelGGL(
+      (dg_operator_kernel<dim, n, NC, T, cart, skew, fin, epi>),
+      dim3((unsigned)blocks),
+      dim3(CPB * nl),
+      0,
+      stream,
+      P,
+      (const DevBasis<T> *)bt,
+      (const DevBasis<double> *)bd,
+      *(const OpBuffers<T> *)io);
+    return hipGetLastError();
+  }
+} // namespace dgops

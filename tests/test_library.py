@@ -1,0 +1,47 @@
+"""CPU checks of the C ABI library: it loads and exports every entry point
+that include/dgops.h declares (no GPU calls)."""
+import ctypes
+import os
+import re
+
+from advection_miniapp_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "dgops.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(dg_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_symbols()
+    assert len(names) > 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # and the Python binding knows all of them
+    assert set(names) == set(_lib.SIGNATURES), (
+        set(names) ^ set(_lib.SIGNATURES))
+
+
+def test_config_default_and_validation_without_gpu():
+    lib = _lib.load()
+    cfg = _lib.DgConfig()
+    lib.dg_config_default(ctypes.byref(cfg))
+    assert cfg.dim == 3 and cfg.degree == 4 and cfg.flux_alpha == 1.0
+    cfg.degree = 11
+    h = ctypes.c_void_p()
+    # rejected before any device call
+    assert lib.dg_create(ctypes.byref(cfg), None, ctypes.byref(h)) == \
+        _lib.DG_ERR_UNSUPPORTED
+    assert b"degree" in lib.dg_last_error()
+    assert lib.dg_lsrk_n_stages(_lib.DG_LSRK_5_4) == 5
+    assert lib.dg_lsrk_n_stages(_lib.DG_LSRK_9_5) == 9
+    assert lib.dg_lsrk_n_stages(17) == -1
+
+
+def test_null_handle_is_an_error():
+    lib = _lib.load()
+    assert lib.dg_apply(None, None, None, 0.0) == _lib.DG_ERR_ARGUMENT

@@ -1,0 +1,257 @@
+"""Pins the CPU oracle against closed forms, invariants and brute force.
+
+The reference ships no tests or golden vectors (it has no test suite), so
+every check here comes from mathematics that fixes the answer."""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import Cfg, gl_nodes, nodal_points, rel_diff
+
+
+@pytest.mark.parametrize("n", range(2, 9))
+def test_basis_1d_against_numpy(n):
+    gp, gw, gl = oracle.basis_1d(n)
+    x, w = np.polynomial.legendre.leggauss(n)
+    assert np.allclose(gp, (x + 1) / 2, atol=1e-15, rtol=0)
+    assert np.allclose(gw, w / 2, atol=1e-15, rtol=0)
+    assert np.allclose(gl, gl_nodes(n), atol=2e-15, rtol=0)
+
+
+def _refine_order(dim, n_coarse, r):
+    """Cell coordinates in the order deal.II's refine_global produces on
+    lexicographic coarse cells: every cell is replaced by its children
+    (child c at offset (c&1, c>>1&1, c>>2&1)), recursively."""
+    cells = []
+    for k in range(n_coarse[2] if dim == 3 else 1):
+        for j in range(n_coarse[1]):
+            for i in range(n_coarse[0]):
+                cells.append((i, j, k))
+    for _ in range(r):
+        new = []
+        for (i, j, k) in cells:
+            for c in range(2 ** dim):
+                new.append((2 * i + (c & 1), 2 * j + ((c >> 1) & 1),
+                            2 * k + ((c >> 2) & 1) if dim == 3 else 0))
+        cells = new
+    return np.array(cells, dtype=np.int32)
+
+
+@pytest.mark.parametrize("dim,nc,r", [(2, [1, 1, 1], 3), (3, [1, 1, 1], 2),
+                                      (3, [3, 1, 2], 1), (2, [3, 3, 1], 2)])
+def test_cell_order_is_refine_global(dim, nc, r):
+    o = Cfg(dim=dim, degree=1, n_coarse=nc, refine=r).oracle_handle()
+    assert np.array_equal(o.cell_coordinates(), _refine_order(dim, nc, r))
+
+
+def test_neighbors_and_periodicity():
+    c = Cfg(dim=3, degree=1, n_coarse=[2, 1, 1], refine=1,
+            periodic=[1, 0, 1])
+    o = c.oracle_handle()
+    ijk = o.cell_coordinates()
+    nb = o.neighbors()
+    N = [4, 2, 2]
+    index = {tuple(v): i for i, v in enumerate(ijk)}
+    for cell, v in enumerate(ijk):
+        for d in range(3):
+            for s in range(2):
+                w = list(v)
+                w[d] += 1 if s else -1
+                if w[d] < 0 or w[d] >= N[d]:
+                    if not c.periodic[d]:
+                        assert nb[cell, 2 * d + s] == -1
+                        continue
+                    w[d] %= N[d]
+                assert nb[cell, 2 * d + s] == index[tuple(w)]
+
+
+def test_perturbed_vertices_seeded_and_bounded():
+    c = Cfg(dim=3, degree=1, refine=3, perturbation=0.1, seed=7)
+    o = c.oracle_handle()
+    v = o.cell_vertices()
+    h = 1.0 / 8
+    ijk = o.cell_coordinates()
+    for cell in range(0, len(ijk), 37):
+        for corner in range(8):
+            ref = (ijk[cell] + [(corner >> d) & 1 for d in range(3)]) * h
+            off = v[cell, corner] - ref
+            assert np.all(np.abs(off) <= 0.1 * h + 1e-15)
+            on_boundary = np.any((ref == 0) | (np.abs(ref - 1) < 1e-15))
+            if on_boundary:
+                assert np.all(off == 0)
+    # the same seed gives the same mesh, another seed another one
+    v2 = Cfg(dim=3, degree=1, refine=3, perturbation=0.1,
+             seed=7).oracle_handle().cell_vertices()
+    v3 = Cfg(dim=3, degree=1, refine=3, perturbation=0.1,
+             seed=8).oracle_handle().cell_vertices()
+    assert np.array_equal(v, v2) and not np.array_equal(v, v3)
+
+
+def _poly_problem(dim, k, pert):
+    a = [0.7, -0.4, 0.3][:dim] + [0.0] * (3 - dim)
+    return Cfg(dim=dim, degree=k, refine=1 if dim == 3 else 2,
+               periodic=[0, 0, 0], perturbation=pert, velocity=a,
+               solution_kind=4, solution_param=[k], lo=[-0.5, 0, 0.25],
+               hi=[0.5, 1.5, 1.0])
+
+
+@pytest.mark.parametrize("dim,k", [(2, 1), (2, 3), (3, 2), (3, 4)])
+def test_known_answer_polynomial(dim, k):
+    """u in Q_k, continuous, Dirichlet data = u: M^-1(-L u) = -a.grad u
+    exactly on affine cells (QGauss(k+1) integrates degree 2k+1)."""
+    c = _poly_problem(dim, k, 0.0)
+    o = c.oracle_handle()
+    u = o.project(0.0)
+    du = o.apply(u, 0.0)
+    pts = nodal_points(o.cell_vertices(), k + 1, dim)
+    a = np.array(c.velocity[:dim])
+    s = 1 + sum((d + 1) * pts[..., d] for d in range(dim))
+    grad_dot = k * s ** (k - 1) * sum((d + 1) * a[d] for d in range(dim))
+    assert rel_diff(du, -grad_dot.ravel()) < 1e-11
+    # the projection itself interpolates the polynomial exactly
+    assert rel_diff(u, (s ** k).ravel()) < 1e-12
+
+
+def test_inverse_mass_is_inverse_of_mass():
+    c = Cfg(dim=3, degree=3, refine=1, perturbation=0.15)
+    o = c.oracle_handle()
+    r = np.random.default_rng(1).standard_normal(o.n_dofs)
+    assert rel_diff(o.mass(o.inverse_mass(r)), r) < 1e-12
+
+
+def test_mass_matrix_brute_force():
+    """Cell mass matrix by direct quadrature with Lagrange polynomials on
+    the (numpy) Gauss-Lobatto nodes, 2D affine cell."""
+    k = 3
+    c = Cfg(dim=2, degree=k, refine=0, lo=[0, 0], hi=[2.0, 0.5],
+            periodic=[0, 0, 0])
+    o = c.oracle_handle()
+    n = k + 1
+    xg = gl_nodes(n)
+    xq, wq = np.polynomial.legendre.leggauss(n)
+    xq, wq = (xq + 1) / 2, wq / 2
+
+    def phi(i, x):
+        return np.prod([(x - xg[j]) / (xg[i] - xg[j]) for j in range(n)
+                        if j != i], axis=0)
+    P = np.array([[phi(i, x) for i in range(n)] for x in xq])  # [q][i]
+    M1 = P.T @ np.diag(wq) @ P
+    M = np.kron(M1 * 0.5, M1 * 2.0)  # y-extent 0.5 (slow), x-extent 2 (fast)
+    u = np.random.default_rng(3).standard_normal(n * n)
+    assert rel_diff(o.mass(u), M @ u) < 1e-13
+
+
+@pytest.mark.parametrize("alpha", [0.0, 1.0])
+def test_conservation(alpha):
+    """Periodic mesh, conservative form: 1^T M du/dt = 0."""
+    c = Cfg(dim=3, degree=3, refine=1, perturbation=0.1, velocity_kind=1,
+            flux_alpha=alpha)
+    o = c.oracle_handle()
+    u = np.random.default_rng(2).standard_normal(o.n_dofs)
+    du = o.loop(u, 0.0)      # = M * (du/dt)
+    assert abs(du.sum()) < 1e-11 * np.abs(du).sum()
+
+
+@pytest.mark.parametrize("alpha", [0.0, 1.0])
+def test_energy_skew_form(alpha):
+    """Skew-symmetric form (factor_skew = 1/2): the volume terms cancel at
+    every quadrature point, so u^T M du/dt = -1/2 sum alpha |a.n| [u]^2:
+    zero for the central flux, negative for upwind."""
+    c = Cfg(dim=3, degree=3, refine=1, perturbation=0.1, velocity_kind=1,
+            flux_alpha=alpha, factor_skew=0.5)
+    o = c.oracle_handle()
+    u = np.random.default_rng(2).standard_normal(o.n_dofs)
+    du = o.loop(u, 0.0)
+    e = u @ du
+    if alpha == 0:
+        assert abs(e) < 1e-12 * np.abs(u * du).sum()
+    else:
+        assert e < -1e-3 * np.abs(u * du).sum()
+
+
+def test_constant_is_steady():
+    c = Cfg(dim=2, degree=4, refine=2, perturbation=0.2, velocity_kind=2,
+            time_factor=1, final_time=8.0)
+    o = c.oracle_handle()
+    u = np.ones(o.n_dofs)
+    # the vortex is divergence free only in the continuous sense; for a
+    # constant state the conservative DG form gives sum of flux jumps = 0
+    # and the volume term int grad v . a; check the mass instead
+    du = o.loop(u, 0.3)
+    assert abs(du.sum()) < 1e-12 * np.abs(du).sum() + 1e-14
+
+
+def test_linearity():
+    c = Cfg(dim=3, degree=2, refine=1, perturbation=0.1, velocity_kind=1)
+    o = c.oracle_handle()
+    rng = np.random.default_rng(5)
+    u, v = rng.standard_normal((2, o.n_dofs))
+    lhs = o.apply(2.0 * u - 3.0 * v, 0.0)
+    rhs = 2.0 * o.apply(u, 0.0) - 3.0 * o.apply(v, 0.0)
+    assert rel_diff(lhs, rhs) < 1e-12
+
+
+def test_mass_energy_of_projection():
+    c = Cfg(dim=2, degree=5, refine=2, solution_kind=2,
+            solution_param=[np.pi, 0.0], lo=[-1, -1], hi=[1, 1])
+    o = c.oracle_handle()
+    u = o.project(0.0)
+    m, e, h1 = o.mass_energy(u)
+    # int sin(pi x) sin(pi y) = 0, int sin^2 sin^2 = 1, int |grad|^2 = 2 pi^2
+    assert abs(m) < 1e-12
+    assert abs(e - 1.0) < 1e-6
+    assert abs(h1 - 2 * np.pi ** 2) < 2e-3 * 2 * np.pi ** 2
+    err, nrm = o.l2_error(u, 0.0)
+    assert abs(nrm - 1.0) < 1e-6 and err < 1e-5
+
+
+def test_radau_iia_closed_form():
+    # s = 2: c = (1/3, 1), A = [[5/12, -1/12], [3/4, 1/4]]
+    A, b, c = oracle.irk_coefficients(2)
+    assert np.allclose(A, [[5 / 12, -1 / 12], [3 / 4, 1 / 4]], atol=1e-15)
+    A, b, c = oracle.irk_coefficients(3)
+    s6 = np.sqrt(6)
+    assert np.allclose(c, [(4 - s6) / 10, (4 + s6) / 10, 1.0], atol=1e-15)
+    assert np.allclose(A.sum(axis=1), c, atol=1e-15)
+    # reference literals (advection_solver_irk_real.cc:365-373) are exact
+    ref = np.array([[0.19681547722366044, -0.06553542585019838,
+                     0.02377097434822015],
+                    [0.3944243147390873, 0.29207341166522843,
+                     -0.04154875212599792],
+                    [0.37640306270046725, 0.5124858261884216, 1. / 9.]])
+    assert np.max(np.abs(A - ref)) < 2e-16
+    # eigenvalues of A^-1 (Radau IIA(3))
+    ev = np.sort_complex(np.linalg.eigvals(np.linalg.inv(A)))
+    assert np.allclose(ev, np.sort_complex(np.array(
+        [3.6378342527, 2.6810828736 + 3.0504301992j,
+         2.6810828736 - 3.0504301992j])), atol=1e-9)
+    # order conditions B(2s-1) for s = 4: sum b c^(k-1) = 1/k
+    A, b, c = oracle.irk_coefficients(4)
+    for kk in range(1, 8):
+        assert abs(b @ c ** (kk - 1) - 1 / kk) < 1e-13
+
+
+def test_lsrk_order_of_accuracy():
+    """LSRK(5,4): the difference to a run with a 16x smaller time step (same
+    spatial discretization) drops by ~2^4 per halving of the time step."""
+    c = Cfg(dim=2, degree=3, refine=2, lo=[-1, -1], hi=[1, 1],
+            velocity=[1.0, 0.15], solution_kind=2, solution_param=[np.pi, 0])
+    o = c.oracle_handle()
+    T = 0.2
+
+    def run(nsteps):
+        dt = T / nsteps
+        u = o.project(0.0)
+        r1, r2 = o.zeros(), o.zeros()
+        t = 0.0
+        for _ in range(nsteps):
+            o.lsrk_time_step(1, t, dt, u, r1, r2)
+            t += dt
+        return u
+    ref = run(256)
+    e1 = np.linalg.norm(run(8) - ref)
+    e2 = np.linalg.norm(run(16) - ref)
+    rate = np.log2(e1 / e2)
+    assert 3.7 < rate < 4.5, (e1, e2, rate)
