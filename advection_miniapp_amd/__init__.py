@@ -6,8 +6,13 @@ of the reference's operator interfaces on top of it.
 """
 from . import _lib
 from ._lib import DgError
-from .operators import (AdvectionOperation, LowStorageRungeKuttaIntegrator,
-                        irk_coefficients, make_config)
+from .operators import (AdvectionOperation,
+                        DiagonallyImplicitRungeKuttaIntegrator,
+                        ImplicitRungeKuttaIntegrator,
+                        LowStorageRungeKuttaIntegrator, irk_coefficients,
+                        make_config, solver_control)
 
 __all__ = ["AdvectionOperation", "LowStorageRungeKuttaIntegrator",
-           "DgError", "make_config", "irk_coefficients", "_lib"]
+           "ImplicitRungeKuttaIntegrator",
+           "DiagonallyImplicitRungeKuttaIntegrator", "DgError",
+           "make_config", "irk_coefficients", "solver_control", "_lib"]

@@ -111,6 +111,10 @@ SIGNATURES = {
                               ctypes.POINTER(DgSolverControl)]),
     "dg_dirk_time_step": (_i, [_vp, _i, _d, _d, _vp,
                                ctypes.POINTER(DgSolverControl)]),
+    "dg_irk_integrator_create": (_i, [_vp, ctypes.POINTER(_vp)]),
+    "dg_irk_integrator_destroy": (_i, [_vp]),
+    "dg_irk_integrator_step": (_i, [_vp, _d, _d, _vp,
+                                    ctypes.POINTER(DgSolverControl), _dp]),
     "dg_vector_dot": (_i, [_vp, _i, _i64, _vp, _vp, _dp]),
     "dg_vector_multi_dot": (_i, [_vp, _i, _i64, _i, _vp, _vp, _dp]),
     "dg_vector_multi_axpy": (_i, [_vp, _i, _i64, _i, _dp, _vp, _vp]),

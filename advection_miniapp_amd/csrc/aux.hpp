@@ -31,6 +31,11 @@ namespace dgops
                      int64_t x_stride, const void *y, double *result);
   void vec_copy_convert(Operator &op, int64_t n, int ps, const void *src,
                         int pd, void *dst);
+  // one pass over four vectors and a right-hand side: the lower triangle of
+  // their Gram matrix (10 values, row by row) and their products with rhs
+  // (4 values), as the fused loop of advection_solver_irk_real.cc:2551-2645
+  void vec_gram4(Operator &op, int64_t n, const double *const v[4],
+                 const double *rhs, double out[14]);
   // sol_out = sol_in + b k; ri_out = sol_in + a k (ri_out may be null)
   void vec_lsrk_update(Operator &op, int p, int64_t n, double b, double a,
                        const void *k, const void *sol_in, void *sol_out,

@@ -454,6 +454,47 @@ namespace dgops
         }
     }
 
+    __global__ void
+    gram4_kernel(int64_t n, const double *__restrict__ v0,
+                 const double *__restrict__ v1, const double *__restrict__ v2,
+                 const double *__restrict__ v3, const double *__restrict__ r,
+                 double *partials)
+    {
+      __shared__ double red[14][vec_threads / 64];
+      double            s[14];
+      for (int j = 0; j < 14; ++j)
+        s[j] = 0;
+      for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+           i += (int64_t)gridDim.x * blockDim.x)
+        {
+          const double a[4] = {v0[i], v1[i], v2[i], v3[i]};
+          const double b    = r[i];
+          int          c    = 0;
+          for (int l = 0; l < 4; ++l)
+            for (int j = 0; j <= l; ++j, ++c)
+              s[c] = fma(a[l], a[j], s[c]);
+          for (int l = 0; l < 4; ++l, ++c)
+            s[c] = fma(a[l], b, s[c]);
+        }
+      const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+      for (int j = 0; j < 14; ++j)
+        {
+          double v = s[j];
+          for (int off = 32; off > 0; off >>= 1)
+            v += __shfl_down(v, off, 64);
+          if (lane == 0)
+            red[j][wave] = v;
+        }
+      __syncthreads();
+      if (threadIdx.x < 14)
+        {
+          double v = 0;
+          for (int w = 0; w < vec_threads / 64; ++w)
+            v += red[threadIdx.x][w];
+          partials[blockIdx.x * 14 + threadIdx.x] = v;
+        }
+    }
+
     template <typename TS, typename TD>
     __global__ void
     convert_kernel(int64_t n, const TS *__restrict__ x, TD *__restrict__ y)
@@ -750,6 +791,19 @@ namespace dgops
     else
       multi_dot_t<double>(op, n, m, (const double *)x, xs, (const double *)y,
                           result);
+  }
+
+  void
+  vec_gram4(Operator &op, int64_t n, const double *const v[4],
+            const double *rhs, double out[14])
+  {
+    const unsigned nb = std::min<unsigned>(grid_for(n), 1024);
+    op.d_partials.alloc(sizeof(double) * nb * 14);
+    hipLaunchKernelGGL(gram4_kernel, dim3(nb), dim3(vec_threads), 0,
+                       op.stream, n, v[0], v[1], v[2], v[3], rhs,
+                       op.d_partials.as<double>());
+    hip_check(hipGetLastError(), "gram4");
+    reduce_to_host(op, nb, 14, out);
   }
 
   void

@@ -17,6 +17,29 @@ namespace dgops
     return m;
   }
 
+  std::map<uint32_t, BurgersLaunch> &
+  burgers_registry()
+  {
+    static std::map<uint32_t, BurgersLaunch> m;
+    return m;
+  }
+
+  // Burgers right-hand side (burgers_solver.cc:259-341) with the fused
+  // LSRK epilogue or a plain store
+  static void
+  launch_burgers_op(const Operator &op, const OpParams &P, int epi,
+                    const void *io)
+  {
+    auto it = burgers_registry().find(
+      burgers_key(op.cfg.dim, op.n, P.cartesian ? 1 : 0, epi));
+    if (it == burgers_registry().end())
+      throw Error(DG_ERR_UNSUPPORTED,
+                  "Burgers operator built for degree 1..4 only");
+    hip_check(it->second(P, op.d_basis_d.ptr, op.d_burgers.ptr, io,
+                         op.stream),
+              "burgers launch");
+  }
+
   namespace
   {
     thread_local std::string last_error;
@@ -188,8 +211,14 @@ namespace dgops
                   int             epi,
                   const void     *io)
   {
+    const int diff = op.cfg.operator_kind == DG_OP_ADVECTION_DIFFUSION;
+    if (diff && !P.cartesian)
+      throw Error(DG_ERR_UNSUPPORTED,
+                  "advection-diffusion (SIPG) is built for axis-aligned "
+                  "meshes (perturbation 0)");
     const uint32_t key = op_key(op.cfg.dim, op.n, nc, fp32 ? 1 : 0,
-                                P.cartesian ? 1 : 0, skew ? 1 : 0, fin, epi);
+                                P.cartesian ? 1 : 0, skew ? 1 : 0, diff, fin,
+                                epi);
     auto it = op_registry().find(key);
     if (it == op_registry().end())
       throw Error(DG_ERR_UNSUPPORTED,
@@ -232,6 +261,12 @@ namespace dgops
       require(c.irk_stages >= 0 && c.irk_stages <= 5, "irk_stages 0..5");
       if (c.velocity_kind == DG_VEL_CELLULAR_3D)
         require(c.dim == 3, "cellular velocity field is 3D");
+      if (c.operator_kind == DG_OP_ADVECTION_DIFFUSION)
+        for (int d = 0; d < c.dim; ++d)
+          require(c.periodic[d],
+                  "advection-diffusion: periodic meshes only (the reference "
+                  "has no boundary faces for SIPG)",
+                  DG_ERR_UNSUPPORTED);
     }
   } // namespace
 } // namespace dgops
@@ -307,6 +342,41 @@ dg_create(const dg_config *cfg, void *stream, dg_operator **out)
     op->nd          = cfg->dim == 3 ? op->n * op->n * op->n : op->n * op->n;
     op->n_dofs      = op->mesh.n_cells * op->nd;
     op->basis       = make_basis(op->n);
+    if (cfg->operator_kind == DG_OP_BURGERS)
+      {
+        // QGauss(k(k+1)/2) tables (burgers_solver.cc:234)
+        require(cfg->degree <= 4, "Burgers operator: degree 1..4",
+                DG_ERR_UNSUPPORTED);
+        require(cfg->precision == DG_FP64, "Burgers operator: FP64",
+                DG_ERR_UNSUPPORTED);
+        for (int d = 0; d < cfg->dim; ++d)
+          require(cfg->periodic[d],
+                  "Burgers operator: periodic meshes only (the reference "
+                  "has no boundary faces)",
+                  DG_ERR_UNSUPPORTED);
+        BurgersTables T = {};
+        const int     m = cfg->degree * (cfg->degree + 1) / 2;
+        T.m             = m;
+        std::vector<double> gp, gw;
+        gauss_points(m, gp, gw);
+        const auto xl = gauss_lobatto_points_ext(op->n);
+        for (int q = 0; q < m; ++q)
+          {
+            T.wm[q] = gw[q];
+            T.xm[q] = gp[q];
+            for (int i = 0; i < op->n; ++i)
+              {
+                real_ext v, dv;
+                lagrange(xl, i, (real_ext)gp[q], v, dv);
+                T.Sm[q][i] = (double)v;
+                T.Gm[q][i] = (double)dv;
+              }
+          }
+        op->d_burgers.alloc(sizeof(T));
+        hip_check(hipMemcpy(op->d_burgers.ptr, &T, sizeof(T),
+                            hipMemcpyHostToDevice),
+                  "upload burgers tables");
+      }
 
     const auto bd = make_dev_basis<double>(op->basis);
     const auto bf = make_dev_basis<float>(op->basis);
@@ -368,11 +438,18 @@ dg_apply(dg_operator *h, const void *src, void *dst, double t)
 {
   return guard([&] {
     Operator *op = get(h);
-    require(op->cfg.operator_kind == DG_OP_ADVECTION,
-            "dg_apply: advection operator only (see dg_burgers/sipg)",
-            DG_ERR_STATE);
     require(src && dst, "null vector");
     require(src != dst, "dg_apply: src and dst must differ");
+    if (op->cfg.operator_kind == DG_OP_BURGERS)
+      {
+        OpParams          P  = op->base_params(t, 1);
+        OpBuffers<double> io = {};
+        io.src   = (const double *)src;
+        io.dst   = (double *)dst;
+        io.scale = 1.;
+        launch_burgers_op(*op, P, epi_store, &io);
+        return;
+      }
     require(op->cfg.factor_skew == 0.,
             "explicit operator is built for factor_skew = 0",
             DG_ERR_UNSUPPORTED);
@@ -408,10 +485,23 @@ namespace dgops
               const void *ri_in, const void *sol_in, void *sol_out,
               void *ri_out, bool last)
   {
+    OpParams P = op.base_params(t, 1);
+    if (op.cfg.operator_kind == DG_OP_BURGERS)
+      {
+        OpBuffers<double> io = {};
+        io.src     = (const double *)ri_in;
+        io.sol_in  = (const double *)sol_in;
+        io.sol_out = (double *)sol_out;
+        io.ri_out  = (double *)ri_out;
+        io.b_dt    = b_dt;
+        io.a_dt    = a_dt;
+        io.last    = last;
+        launch_burgers_op(op, P, epi_lsrk, &io);
+        return;
+      }
     require(op.cfg.factor_skew == 0.,
             "explicit operator is built for factor_skew = 0",
             DG_ERR_UNSUPPORTED);
-    OpParams P = op.base_params(t, 1);
     if (op.fp32())
       {
         OpBuffers<float> io = {};
@@ -448,8 +538,6 @@ dg_perform_stage(dg_operator *h, double t, double b_dt, double a_dt,
 {
   return guard([&] {
     Operator *op = get(h);
-    require(op->cfg.operator_kind == DG_OP_ADVECTION,
-            "perform_stage: advection operator only", DG_ERR_STATE);
     require(current_ri && vec_ki && solution && next_ri, "null vector");
     const bool last = a_dt == 0.;
     if (current_ri != solution && current_ri != next_ri)
@@ -463,7 +551,8 @@ dg_perform_stage(dg_operator *h, double t, double b_dt, double a_dt,
     require(vec_ki != current_ri,
             "perform_stage: vec_ki must not alias current_ri");
     require(dg_apply(h, current_ri, vec_ki, t) == DG_OK, last_error);
-    vec_lsrk_update(*op, op->cfg.precision, op->n_dofs, b_dt, a_dt, vec_ki,
+    vec_lsrk_update(*op, op->cfg.precision,
+                    op->n_dofs * op->cfg.n_components, b_dt, a_dt, vec_ki,
                     solution, solution, last ? nullptr : next_ri);
   });
 }
@@ -489,8 +578,6 @@ dg_lsrk_time_step(dg_operator *h, int scheme, double t, double dt,
 {
   return guard([&] {
     Operator *op = get(h);
-    require(op->cfg.operator_kind == DG_OP_ADVECTION,
-            "lsrk_time_step: advection operator only", DG_ERR_STATE);
     require(solution && vec_ri && vec_ki, "null vector");
     require(solution != vec_ri && solution != vec_ki && vec_ri != vec_ki,
             "the three registers must be distinct");

@@ -9,7 +9,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 OUT = os.path.join(HERE, "generated")
 
-# (dim, n, nc, fp32, cart, skew, fin, epi)
+# operator variant: (dim, n, nc, fp32, cart, skew, diff, fin, epi)
 FIN_INVMASS, FIN_INTEGRATE = 0, 1
 EPI_STORE, EPI_LSRK = 0, 1
 
@@ -19,18 +19,27 @@ def op_variants():
     # explicit right-hand side M^-1(-L u): apply and fused LSRK stage
     for dim, n, cart, epi in itertools.product(
             (2, 3), range(2, 9), (0, 1), (EPI_STORE, EPI_LSRK)):
-        v.append((dim, n, 1, 0, cart, 0, FIN_INVMASS, epi))
+        v.append((dim, n, 1, 0, cart, 0, 0, FIN_INVMASS, epi))
+    # fp32 explicit right-hand side on axis-aligned meshes
+    for dim, n in itertools.product((2, 3), range(2, 9)):
+        v.append((dim, n, 1, 1, 1, 0, 0, FIN_INVMASS, EPI_STORE))
     # implicit single-stage operator (M/dt + L) and IRK right-hand side,
     # skew-symmetric form, fp64 and fp32
     for dim, n, cart, fp32 in itertools.product(
             (2, 3), range(2, 9), (0, 1), (0, 1)):
-        v.append((dim, n, 1, fp32, cart, 1, FIN_INTEGRATE, EPI_STORE))
+        v.append((dim, n, 1, fp32, cart, 1, 0, FIN_INTEGRATE, EPI_STORE))
     # stage-coupled IRK operator
     for dim, n, nc, fp32 in itertools.product(
             (2, 3), range(2, 8), (2, 3, 4), (0, 1)):
-        v.append((dim, n, nc, fp32, 1, 1, FIN_INTEGRATE, EPI_STORE))
+        v.append((dim, n, nc, fp32, 1, 1, 0, FIN_INTEGRATE, EPI_STORE))
         if dim == 3 and n == 5:
-            v.append((dim, n, nc, fp32, 0, 1, FIN_INTEGRATE, EPI_STORE))
+            v.append((dim, n, nc, fp32, 0, 1, 0, FIN_INTEGRATE, EPI_STORE))
+    # advection + SIPG diffusion, axis-aligned: explicit (apply, LSRK stage)
+    # and implicit operator, fp64 and fp32
+    for dim, n, fp32 in itertools.product((2, 3), range(2, 9), (0, 1)):
+        v.append((dim, n, 1, fp32, 1, 0, 1, FIN_INVMASS, EPI_STORE))
+        v.append((dim, n, 1, fp32, 1, 0, 1, FIN_INVMASS, EPI_LSRK))
+        v.append((dim, n, 1, fp32, 1, 1, 1, FIN_INTEGRATE, EPI_STORE))
     return v
 
 
@@ -46,9 +55,27 @@ FDM_TYPES = {0: ("double", "double", "double"), 1: ("float", "float", "float"),
              2: ("float", "double", "double")}
 
 
+def burgers_variants():
+    v = []
+    for dim, k, cart, epi in itertools.product((2, 3), (1, 2, 3, 4), (0, 1),
+                                               (EPI_STORE, EPI_LSRK)):
+        v.append(("burgers", dim, k + 1, k * (k + 1) // 2, cart, epi))
+    return v
+
+
+def b(x):
+    return "true" if x else "false"
+
+
 def emit(path, variants):
     lines = ['#include "../registry.hpp"', "", "namespace dgops", "{"]
     for i, var in enumerate(variants):
+        if var[0] == "burgers":
+            _, dim, n, m, cart, epi = var
+            lines.append(
+                f"  static BurgersRegistrar g{i}(burgers_key({dim}, {n}, {cart}, "
+                f"{epi}), &launch_burgers<{dim}, {n}, {m}, {b(cart)}, {epi}>);")
+            continue
         if var[0] == "fdm":
             _, dim, n, ns, prec = var
             tc, ti, to = FDM_TYPES[prec]
@@ -56,13 +83,12 @@ def emit(path, variants):
                 f"  static FdmRegistrar f{i}(fdm_key({dim}, {n}, {ns}, {prec}), "
                 f"&launch_fdm<{dim}, {n}, {ns}, {tc}, {ti}, {to}>);")
             continue
-        dim, n, nc, fp32, cart, skew, fin, epi = var
+        dim, n, nc, fp32, cart, skew, diff, fin, epi = var
         T = "float" if fp32 else "double"
         lines.append(
             f"  static OpRegistrar r{i}(op_key({dim}, {n}, {nc}, {fp32}, {cart}, "
-            f"{skew}, {fin}, {epi}), &launch_op<{dim}, {n}, {nc}, {T}, "
-            f"{'true' if cart else 'false'}, {'true' if skew else 'false'}, "
-            f"{fin}, {epi}>);")
+            f"{skew}, {diff}, {fin}, {epi}), &launch_op<{dim}, {n}, {nc}, {T}, "
+            f"{b(cart)}, {b(skew)}, {b(diff)}, {fin}, {epi}>);")
     lines += ["} // namespace dgops", ""]
     text = "\n".join(lines)
     if os.path.exists(path) and open(path).read() == text:
@@ -71,13 +97,16 @@ def emit(path, variants):
         f.write(text)
 
 
-def main(n_files=16):
+def main(n_files=24):
     os.makedirs(OUT, exist_ok=True)
+
     def weight(v):
         if v[0] == "fdm":
             return v[2] ** (v[1] + 1) * v[3]
-        return v[1] ** (v[0] + 1) * v[2]
-    variants = op_variants() + fdm_variants()
+        if v[0] == "burgers":
+            return v[3] ** (v[1] + 1)
+        return v[1] ** (v[0] + 1) * v[2] * (2 if v[6] else 1)
+    variants = op_variants() + fdm_variants() + burgers_variants()
     # spread heavy (3D, large n) variants evenly
     variants.sort(key=lambda v: -weight(v))
     groups = [[] for _ in range(n_files)]
@@ -88,6 +117,11 @@ def main(n_files=16):
         name = os.path.join(OUT, f"op_{g:02d}.hip")
         emit(name, vs)
         names.append(name)
+    # drop files of an earlier, larger split
+    for f in os.listdir(OUT):
+        p = os.path.join(OUT, f)
+        if f.endswith(".hip") and p not in names:
+            os.remove(p)
     return names
 
 

@@ -1,0 +1,25 @@
+#include "../registry.hpp"
+
+namespace dgops
+{
+  static FdmRegistrar f0(fdm_key(3, 8, 2, 0), &launch_fdm<3, 8, 2, double, double, double>);
+  static OpRegistrar r1(op_key(3, 8, 1, 0, 0, 0, 0, 0, 0), &launch_op<3, 8, 1, double, false, false, false, 0, 0>);
+  static OpRegistrar r2(op_key(3, 6, 1, 1, 1, 1, 1, 1, 0), &launch_op<3, 6, 1, float, true, true, true, 1, 0>);
+  static FdmRegistrar f3(fdm_key(2, 8, 4, 1), &launch_fdm<2, 8, 4, float, float, float>);
+  static OpRegistrar r4(op_key(3, 6, 1, 0, 1, 1, 0, 1, 0), &launch_op<3, 6, 1, double, true, true, false, 1, 0>);
+  static FdmRegistrar f5(fdm_key(2, 7, 3, 0), &launch_fdm<2, 7, 3, double, double, double>);
+  static FdmRegistrar f6(fdm_key(2, 6, 4, 1), &launch_fdm<2, 6, 4, float, float, float>);
+  static OpRegistrar r7(op_key(3, 5, 1, 0, 0, 0, 0, 0, 1), &launch_op<3, 5, 1, double, false, false, false, 0, 1>);
+  static OpRegistrar r8(op_key(3, 4, 1, 0, 1, 1, 1, 1, 0), &launch_op<3, 4, 1, double, true, true, true, 1, 0>);
+  static FdmRegistrar f9(fdm_key(2, 6, 2, 1), &launch_fdm<2, 6, 2, float, float, float>);
+  static OpRegistrar r10(op_key(3, 4, 1, 0, 0, 0, 0, 0, 0), &launch_op<3, 4, 1, double, false, false, false, 0, 0>);
+  static OpRegistrar r11(op_key(2, 5, 1, 1, 1, 1, 1, 1, 0), &launch_op<2, 5, 1, float, true, true, true, 1, 0>);
+  static BurgersRegistrar g12(burgers_key(2, 4, 1, 1), &launch_burgers<2, 4, 6, true, 1>);
+  static OpRegistrar r13(op_key(2, 4, 1, 1, 1, 1, 1, 1, 0), &launch_op<2, 4, 1, float, true, true, true, 1, 0>);
+  static OpRegistrar r14(op_key(3, 3, 1, 0, 1, 0, 0, 0, 1), &launch_op<3, 3, 1, double, true, false, false, 0, 1>);
+  static OpRegistrar r15(op_key(2, 4, 1, 1, 0, 1, 0, 1, 0), &launch_op<2, 4, 1, float, false, true, false, 1, 0>);
+  static FdmRegistrar f16(fdm_key(3, 2, 3, 0), &launch_fdm<3, 2, 3, double, double, double>);
+  static OpRegistrar r17(op_key(2, 3, 1, 0, 0, 1, 0, 1, 0), &launch_op<2, 3, 1, double, false, true, false, 1, 0>);
+  static OpRegistrar r18(op_key(3, 2, 1, 1, 1, 1, 0, 1, 0), &launch_op<3, 2, 1, float, true, true, false, 1, 0>);
+  static FdmRegistrar f19(fdm_key(2, 2, 1, 0), &launch_fdm<2, 2, 1, double, double, double>);
+} // namespace dgops

@@ -298,8 +298,9 @@ namespace dgops
     vmult_single(Operator &op, bool fp32, double t, double dt, const void *src,
                  void *dst, double scale = 1., bool with_mass = true)
     {
-      require(op.cfg.operator_kind == DG_OP_ADVECTION,
-              "implicit operator: advection only", DG_ERR_STATE);
+      require(op.cfg.operator_kind != DG_OP_BURGERS,
+              "implicit operator: advection (+ diffusion) only",
+              DG_ERR_STATE);
       OpParams P = op.base_params(t + dt, 1);
       P.sign     = 1.;
       P.has_mass = (with_mass && dt != 0.) ? 1 : 0;
@@ -913,6 +914,174 @@ dg_dirk_time_step(dg_operator *h, int n_stages, double time, double dt,
         vec_axpby(*op, p, n, -b[stage] / gamma, ki[stage], 1., solution);
       }
     ctl->iterations = total_it;
+  });
+}
+
+} // extern "C"
+
+// ---- Radau IIA integrator with extrapolated initial guesses ----------------
+
+namespace dgops
+{
+  struct IrkIntegrator
+  {
+    Operator *op;
+    int       s;
+    int64_t   N; // s * n_dofs
+    DevBuf    rhs;
+    DevBuf    sol[4], mv[4];
+    int       sol_idx[4] = {0, 1, 2, 3}; // stage_sol[k] -> buffer
+    int       mv_idx[4]  = {0, 1, 2, 3};
+    unsigned  timestep_number      = 1;
+    double    last_refresh_quality = 1;
+    unsigned  steps_since_refresh  = 0;
+    bool      refresh              = true;
+    double    project_sol[4]       = {0, 0, 0, 0};
+
+    double *stage_sol(int k) { return sol[sol_idx[k]].as<double>(); }
+    double *stage_mv(int k) { return mv[mv_idx[k]].as<double>(); }
+  };
+} // namespace dgops
+
+extern "C" {
+
+int
+dg_irk_integrator_create(dg_operator *h, dg_irk_integrator **out)
+{
+  return guard([&] {
+    Operator *op = get(h);
+    require(out != nullptr, "null output");
+    require(!op->fp32(), "IRK integrator: FP64 operator", DG_ERR_STATE);
+    setup_irk(*op);
+    auto it = std::make_unique<IrkIntegrator>();
+    it->op  = op;
+    it->s   = op->irk.s;
+    it->N   = op->n_dofs * it->s;
+    it->rhs.alloc(8 * it->N);
+    for (int k = 0; k < 4; ++k)
+      {
+        it->sol[k].alloc(8 * it->N);
+        it->mv[k].alloc(8 * it->N);
+        hip_check(hipMemsetAsync(it->sol[k].ptr, 0, 8 * it->N, op->stream),
+                  "memset");
+        hip_check(hipMemsetAsync(it->mv[k].ptr, 0, 8 * it->N, op->stream),
+                  "memset");
+      }
+    *out = reinterpret_cast<dg_irk_integrator *>(it.release());
+  });
+}
+
+int
+dg_irk_integrator_destroy(dg_irk_integrator *it)
+{
+  return guard([&] { delete reinterpret_cast<IrkIntegrator *>(it); });
+}
+
+int
+dg_irk_integrator_step(dg_irk_integrator *hi, double time, double dt,
+                       void *solution, dg_solver_control *ctl,
+                       double *project_out)
+{
+  return guard([&] {
+    require(hi && solution && ctl, "null argument");
+    IrkIntegrator &I  = *reinterpret_cast<IrkIntegrator *>(hi);
+    Operator      &op = *I.op;
+    dg_operator   *h  = reinterpret_cast<dg_operator *>(&op);
+    op.time           = time;
+    op.time_step      = dt;
+    int rc = dg_irk_compute_rhs(h, I.rhs.ptr, solution);
+    if (rc != DG_OK)
+      throw Error(rc, dg_last_error());
+    constexpr int nv = 4;
+    if (I.refresh)
+      {
+        // least-squares fit of rhs by A x_{n-j} (advection_solver_irk_real.cc
+        // :2544-2695)
+        const unsigned n_max =
+          I.timestep_number > nv ? nv : I.timestep_number - 1;
+        for (unsigned st = 0; st < n_max; ++st)
+          vmult_irk(op, false, time, dt, I.stage_sol(st), I.stage_mv(st));
+        const double *v[4] = {I.stage_mv(0), I.stage_mv(1), I.stage_mv(2),
+                              I.stage_mv(3)};
+        double sums[14];
+        vec_gram4(op, I.N, v, I.rhs.as<double>(), sums);
+        double P[nv][nv] = {};
+        for (int i = 0, c = 0; i < nv; ++i)
+          for (int j = 0; j <= i; ++j, ++c)
+            P[i][j] = sums[c];
+        int i = 0;
+        for (; i < nv; ++i)
+          {
+            for (int j = 0; j < i; ++j)
+              {
+                const double inv_entry = P[i][j] / P[j][j];
+                for (int k = j + 1; k <= i; ++k)
+                  P[i][k] -= P[k][j] * inv_entry;
+              }
+            if (P[i][i] < 1e-12 * P[0][0] || P[0][0] < 1e-30)
+              break;
+          }
+        for (int st = 0; st < i; ++st)
+          {
+            I.project_sol[st] = sums[nv * (nv + 1) / 2 + st];
+            for (int j = 0; j < st; ++j)
+              I.project_sol[st] -= P[st][j] / P[j][j] * I.project_sol[j];
+          }
+        for (int st = i; st < nv; ++st)
+          I.project_sol[st] = 0.;
+        for (int st = i - 1; st >= 0; --st)
+          {
+            double sum = I.project_sol[st];
+            for (int j = st + 1; j < i; ++j)
+              sum -= I.project_sol[j] * P[j][st];
+            I.project_sol[st] = sum / P[st][st];
+          }
+      }
+    // extrapolate into the oldest slot, then rotate it to the front
+    {
+      double *dst3 = I.stage_sol(3);
+      // dst3 = p3 x3 + p0 x0 + p1 x1 + p2 x2 (x3 is read before written)
+      vec_axpby(op, DG_FP64, I.N, 0., dst3, I.project_sol[3], dst3);
+      const double w[3] = {I.project_sol[0], I.project_sol[1],
+                           I.project_sol[2]};
+      for (int k = 0; k < 3; ++k)
+        vec_axpby(op, DG_FP64, I.N, w[k], I.stage_sol(k), 1., dst3);
+      std::swap(I.sol_idx[2], I.sol_idx[3]);
+      std::swap(I.sol_idx[1], I.sol_idx[2]);
+      std::swap(I.sol_idx[0], I.sol_idx[1]);
+    }
+    // solve from the extrapolated guess, SolverControl(max, tol * |rhs|)
+    dg_solver_control c = *ctl;
+    rc = dg_irk_solve(h, I.stage_sol(0), I.rhs.ptr, &c);
+    ctl->iterations       = c.iterations;
+    ctl->initial_residual = c.initial_residual;
+    ctl->final_residual   = c.final_residual;
+    if (rc != DG_OK)
+      throw Error(rc, dg_last_error());
+    // refresh heuristic (:2758-2773)
+    if (I.refresh)
+      {
+        I.last_refresh_quality = c.initial_residual;
+        if (I.timestep_number > 4)
+          {
+            I.refresh             = false;
+            I.steps_since_refresh = 0;
+          }
+      }
+    else
+      {
+        if (I.last_refresh_quality < 0.5 * c.initial_residual ||
+            I.steps_since_refresh > 20)
+          I.refresh = true;
+        ++I.steps_since_refresh;
+      }
+    rc = dg_irk_update_solution(h, solution, I.stage_sol(0));
+    if (rc != DG_OK)
+      throw Error(rc, dg_last_error());
+    ++I.timestep_number;
+    if (project_out)
+      for (int k = 0; k < 4; ++k)
+        project_out[k] = I.project_sol[k];
   });
 }
 

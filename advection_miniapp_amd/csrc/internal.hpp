@@ -106,6 +106,7 @@ namespace dgops
     IrkData irk;
     FdmTables1D fdm[2]; // velocity sign + and -
     DevBuf  d_fdm_d, d_fdm_f, d_cell_velocity, d_cell_invdet;
+    DevBuf  d_burgers; // over-integration tables of the Burgers operator
 
     // scratch for reductions and solvers
     DevBuf d_partials, d_result;

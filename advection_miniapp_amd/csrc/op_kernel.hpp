@@ -366,7 +366,7 @@ namespace dgops
   // advection_solver_implicit_real.cc:1032-1058:
   //   F = 1/2 [(u- + u+) an + alpha |an| (u- - u+)] - skew an u-
   // with an = a . n dA on this cell's outward normal.
-  template <int dim, int n, int dir, typename T, bool cartesian>
+  template <int dim, int n, int dir, typename T, bool cartesian, bool diff>
   __device__ __forceinline__ void
   face_terms(const OpParams &P,
              const DevBasis<double> *Bd,
@@ -376,6 +376,7 @@ namespace dgops
              bool                    active,
              const int (&nb)[2 * dim],
              const T (&uplus)[2 * dim],
+             const T (&dplus)[2 * dim],
              const T (&uq)[n],
              T (&acc)[n],
              T    vf,
@@ -418,7 +419,63 @@ namespace dgops
 #pragma unroll
         for (int i = 0; i < n; ++i)
           acc[i] = fma(l[i], sf, acc[i]);
+        if constexpr (diff)
+          {
+            // SIPG (advection_diffusion_solver.cc:569-613) on this cell's
+            // outward normal, axis-aligned cells: with
+            //   jump = -nu/2 (u- - u+), avg = -nu/2 (d_n u- + d_n u+)
+            // the value term is -(jump sigma - avg) and the normal
+            // derivative of the test function is weighted with jump
+            const T(&dl)[max_n] = s ? B.dl1 : B.dl0;
+            const T hd  = (T)P.h[dir];
+            const T sgn = s ? (T)1 : (T)-1;
+            T       dm  = dl[0] * uq[0];
+#pragma unroll
+            for (int i = 1; i < n; ++i)
+              dm = fma(dl[i], uq[i], dm);
+            const T dnm = sgn * dm / hd, dnp = sgn * dplus[f] / hd;
+            T       w   = (T)1;
+            for (int e = 0; e < dim; ++e)
+              if (e != dir)
+                w *= (T)(Bd->w[q[e]] * P.h[e]);
+            const T nu    = (T)P.viscosity;
+            const T sigma = (T)P.penalty / hd;
+            const T jump  = (T)-0.5 * nu * (um - up);
+            const T avg   = (T)-0.5 * nu * (dnm + dnp);
+            const T fv    = (T)P.sign * w * (avg - jump * sigma);
+            const T fd    = (T)P.sign * w * jump * sgn / hd;
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+              acc[i] = fma(l[i], fv, fma(dl[i], fd, acc[i]));
+          }
       }
+  }
+
+  // cell diffusion term along direction dir on axis-aligned cells:
+  // acc += D^T (sign nu |K|/h_d^2 w_q D u_q)  (int grad v . nu grad u)
+  template <int dim, int n, int dir, typename T>
+  __device__ __forceinline__ void
+  diffusion_cell_term(const OpParams &P, const DevBasis<double> *Bd,
+                      const DevBasis<T> &B, int p, const T (&uq)[n],
+                      T (&acc)[n])
+  {
+    double vol = P.h[0] * P.h[1];
+    if (dim == 3)
+      vol *= P.h[2];
+    const T c0 = (T)(P.sign * P.viscosity * vol / (P.h[dir] * P.h[dir]));
+    T       du[n], g[n];
+    mat_apply<n>(B.D, uq, du);
+#pragma unroll
+    for (int i = 0; i < n; ++i)
+      {
+        int q[3];
+        Lay<dim, n>::template idx<dir>(p, i, q);
+        T w = (T)(Bd->w[q[0]] * Bd->w[q[1]]);
+        if (dim == 3)
+          w *= (T)Bd->w[q[2]];
+        g[i] = c0 * w * du[i];
+      }
+    mat_apply_t_add<n>(B.D, g, acc);
   }
 
   // ---- the kernel ------------------------------------------------------------
@@ -452,20 +509,23 @@ namespace dgops
 
   // y_c = fin( sign * L_c u_c + sum_b mass[c][b] M u_b ), c < NC, with
   // L_c the advection operator at velocity vel_factor[c] * a(x).
+  // With `diff` (axis-aligned cells) L also contains the SIPG diffusion
+  // operator of advection_diffusion_solver.cc:531-613.
   template <int dim, int n, int NC, typename T, bool cartesian, bool skew,
-            int fin, int epi>
+            bool diff, int fin, int epi>
   __global__ void __launch_bounds__(256)
   dg_operator_kernel(const OpParams                       P,
                      const DevBasis<T> *__restrict__      Bp,
                      const DevBasis<double> *__restrict__ Bd,
                      const OpBuffers<T>                   io)
   {
+    static_assert(!diff || cartesian, "SIPG needs axis-aligned cells");
     using L            = Lay<dim, n>;
     constexpr int nl   = L::nl;
     constexpr int nd   = L::nd;
     constexpr int CPB  = cells_per_block<dim, n>();
     constexpr int STR  = L::stride;
-    constexpr int FSTR = 2 * dim * nl;
+    constexpr int FSTR = (diff ? 2 : 1) * 2 * dim * nl;
     constexpr int NV   = 1 << dim;
     constexpr int LAST = dim - 1;
     constexpr int NX   = cartesian ? dim : NV * dim;
@@ -507,6 +567,7 @@ namespace dgops
 
     // ---- per component: neighbour traces and values at quadrature points
     T uplus[NC][2 * dim];
+    T dplus[NC][2 * dim];
     T uqs[NC][n];
 #pragma unroll
     for (int c = 0; c < NC; ++c)
@@ -539,6 +600,36 @@ namespace dgops
                       }
                   }
                 f_l[f * nl + j * n + aq] = v;
+                if constexpr (diff)
+                  {
+                    // reference-space derivative of the neighbour normal
+                    // to the face, at its face (nodal derivative at 1 or 0)
+                    const T(&dp)[max_n] = s ? B.dphi0 : B.dphi1;
+                    T         dv        = 0;
+                    if (active && nb[f] >= 0)
+                      {
+                        const T *nsrc = src + (int64_t)nb[f] * nd;
+#pragma unroll
+                        for (int i = 0; i < n; ++i)
+                          {
+                            T dd = 0;
+#pragma unroll
+                            for (int l = 0; l < n; ++l)
+                              {
+                                int ijk[3];
+                                ijk[d]  = l;
+                                ijk[t1] = i;
+                                ijk[t2] = j;
+                                dd      = fma(dp[l],
+                                         nsrc[ijk[0] + n * ijk[1] +
+                                              n * n * ijk[2]],
+                                         dd);
+                              }
+                            dv = fma(B.S[aq][i], dd, dv);
+                          }
+                      }
+                    f_l[(6 + f) * nl + j * n + aq] = dv;
+                  }
               }
           }
         // own values, sweep along x
@@ -571,6 +662,15 @@ namespace dgops
                 for (int jj = 0; jj < n; ++jj)
                   v = fma(B.S[bq][jj], f_l[f * nl + jj * n + aq], v);
                 uplus[c][f] = v;
+                if constexpr (diff)
+                  {
+                    T dv = 0;
+#pragma unroll
+                    for (int jj = 0; jj < n; ++jj)
+                      dv = fma(B.S[bq][jj], f_l[(6 + f) * nl + jj * n + aq],
+                               dv);
+                    dplus[c][f] = dv;
+                  }
               }
           }
         else
@@ -579,20 +679,32 @@ namespace dgops
             for (int f = 0; f < 4; ++f)
               {
                 const int d = f / 2, s = f % 2;
-                T         v = 0;
+                T         v = 0, dv = 0;
                 if (active && nb[f] >= 0)
                   {
                     const T  *nsrc = src + (int64_t)nb[f] * nd;
                     const int nn   = s ? 0 : n - 1;
+                    const T(&dp)[max_n] = s ? B.dphi0 : B.dphi1;
 #pragma unroll
                     for (int i = 0; i < n; ++i)
                       {
                         const int ij0 = d == 0 ? nn : i;
                         const int ij1 = d == 0 ? i : nn;
                         v = fma(B.S[p][i], nsrc[ij0 + n * ij1], v);
+                        if constexpr (diff)
+                          {
+                            T dd = 0;
+#pragma unroll
+                            for (int l = 0; l < n; ++l)
+                              dd = fma(dp[l],
+                                       nsrc[d == 0 ? l + n * i : i + n * l],
+                                       dd);
+                            dv = fma(B.S[p][i], dd, dv);
+                          }
                       }
                   }
                 uplus[c][f] = v;
+                dplus[c][f] = dv;
               }
           }
         lds_read_line<dim, n, 1>(acc_l, p, t);
@@ -664,9 +776,12 @@ namespace dgops
               for (int i = 0; i < n; ++i)
                 acc[i] = fma(fs * cv[LAST][i], du[i], acc[i]);
             }
-          face_terms<dim, n, LAST, T, cartesian>(P, Bd, B, X, p, active, nb,
-                                                 uplus[c], uqs[c], acc, vf,
-                                                 fan, c == 0);
+          if constexpr (diff)
+            diffusion_cell_term<dim, n, LAST, T>(P, Bd, B, p, uqs[c], acc);
+          face_terms<dim, n, LAST, T, cartesian, diff>(P, Bd, B, X, p, active,
+                                                       nb, uplus[c], dplus[c],
+                                                       uqs[c], acc, vf, fan,
+                                                       c == 0);
           lds_write_line<dim, n, LAST>(u_l, p, uqs[c]);
           lds_write_line<dim, n, LAST>(acc_l, p, acc);
 #pragma unroll
@@ -715,9 +830,12 @@ namespace dgops
                 for (int i = 0; i < n; ++i)
                   acc[i] = fma(sk[i], du[i], acc[i]);
               }
-            face_terms<dim, n, 1, T, cartesian>(P, Bd, B, X, p, active, nb,
-                                                uplus[c], uq, acc, vf, fan,
-                                                c == 0);
+            if constexpr (diff)
+              diffusion_cell_term<dim, n, 1, T>(P, Bd, B, p, uq, acc);
+            face_terms<dim, n, 1, T, cartesian, diff>(P, Bd, B, X, p, active,
+                                                      nb, uplus[c], dplus[c],
+                                                      uq, acc, vf, fan,
+                                                      c == 0);
             lds_write_line<dim, n, 1>(acc_l, p, acc);
             __syncthreads();
           }
@@ -738,9 +856,11 @@ namespace dgops
               for (int i = 0; i < n; ++i)
                 acc[i] = fma(sk[i], du[i], acc[i]);
             }
-          face_terms<dim, n, 0, T, cartesian>(P, Bd, B, X, p, active, nb,
-                                              uplus[c], uq, acc, vf, fan,
-                                              c == 0);
+          if constexpr (diff)
+            diffusion_cell_term<dim, n, 0, T>(P, Bd, B, p, uq, acc);
+          face_terms<dim, n, 0, T, cartesian, diff>(P, Bd, B, X, p, active,
+                                                    nb, uplus[c], dplus[c], uq,
+                                                    acc, vf, fan, c == 0);
         }
         __syncthreads();
         if constexpr (fin == fin_inverse_mass)

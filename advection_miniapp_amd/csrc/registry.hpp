@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <map>
 
+#include "burgers_kernel.hpp"
 #include "fdm_kernel.hpp"
 #include "op_kernel.hpp"
 
@@ -20,11 +21,13 @@ namespace dgops
                                   hipStream_t);
 
   inline uint32_t
-  op_key(int dim, int n, int nc, int fp32, int cart, int skew, int fin, int epi)
+  op_key(int dim, int n, int nc, int fp32, int cart, int skew, int diff,
+         int fin, int epi)
   {
     return (uint32_t)dim | (uint32_t)n << 2 | (uint32_t)nc << 6 |
            (uint32_t)fp32 << 10 | (uint32_t)cart << 11 |
-           (uint32_t)skew << 12 | (uint32_t)fin << 13 | (uint32_t)epi << 15;
+           (uint32_t)skew << 12 | (uint32_t)fin << 13 | (uint32_t)epi << 15 |
+           (uint32_t)diff << 17;
   }
 
   std::map<uint32_t, OpLaunch> &op_registry();
@@ -81,8 +84,54 @@ namespace dgops
     return hipGetLastError();
   }
 
-  template <int dim, int n, int NC, typename T, bool cart, bool skew, int fin,
-            int epi>
+  // Burgers right-hand side variants
+  using BurgersLaunch = hipError_t (*)(const OpParams &,
+                                       const void *basis_d,
+                                       const void *tables,
+                                       const void *io,
+                                       hipStream_t);
+
+  inline uint32_t
+  burgers_key(int dim, int n, int cart, int epi)
+  {
+    return (uint32_t)dim | (uint32_t)n << 2 | (uint32_t)cart << 6 |
+           (uint32_t)epi << 7 | 1u << 21;
+  }
+
+  std::map<uint32_t, BurgersLaunch> &burgers_registry();
+
+  struct BurgersRegistrar
+  {
+    BurgersRegistrar(uint32_t key, BurgersLaunch f)
+    {
+      burgers_registry()[key] = f;
+    }
+  };
+
+  template <int dim, int n, int m, bool cart, int epi>
+  hipError_t
+  launch_burgers(const OpParams &P,
+                 const void     *bd,
+                 const void     *tables,
+                 const void     *io,
+                 hipStream_t     stream)
+  {
+    if (P.n_cells <= 0)
+      return hipSuccess;
+    hipLaunchKernelGGL((burgers_kernel<dim, n, m, double, cart, epi>),
+                       dim3((unsigned)P.n_cells),
+                       dim3(256),
+                       0,
+                       stream,
+                       P,
+                       (const DevBasis<double> *)bd,
+                       (const BurgersTables *)tables,
+                       *(const OpBuffers<double> *)io);
+    return hipGetLastError();
+  }
+
+  template <int dim, int n, int NC, typename T, bool cart, bool skew,
+            bool diff, int fin, int epi>
   hipError_t
   launch_op(const OpParams &P,
             const void     *bt,
@@ -96,7 +145,7 @@ namespace dgops
     if (blocks <= 0)
       return hipSuccess;
     hipLaunchKernelGGL(
-      (dg_operator_kernel<dim, n, NC, T, cart, skew, fin, epi>),
+      (dg_operator_kernel<dim, n, NC, T, cart, skew, diff, fin, epi>),
       dim3((unsigned)blocks),
       dim3(CPB * nl),
       0,

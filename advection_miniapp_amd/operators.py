@@ -210,6 +210,66 @@ class AdvectionOperation:
         return a
 
 
+def solver_control(max_iterations=400, rel_tolerance=1e-8, abs_tolerance=0.0,
+                   restart=10, flexible=True, prec_precision=L.DG_FP64,
+                   inner_iterations=0):
+    """dg_solver_control with the reference's defaults
+    (SolverControl(400, 1e-8 |rhs|), max_basis_size 10, FGMRES)."""
+    c = L.DgSolverControl()
+    c.max_iterations = max_iterations
+    c.rel_tolerance = rel_tolerance
+    c.abs_tolerance = abs_tolerance
+    c.restart = restart
+    c.flexible = 1 if flexible else 0
+    c.prec_precision = prec_precision
+    c.inner_iterations = inner_iterations
+    return c
+
+
+class ImplicitRungeKuttaIntegrator:
+    """Radau IIA stepping of advection_solver_irk_real.cc:2484-2779 with the
+    extrapolated initial guess of the last four stage solutions."""
+
+    def __init__(self, operator):
+        self.op = operator
+        h = ctypes.c_void_p()
+        L.check(operator._lib.dg_irk_integrator_create(operator._h,
+                                                       ctypes.byref(h)))
+        self._h = h
+        self.project_sol = np.zeros(4)
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self.op._lib.dg_irk_integrator_destroy(self._h)
+            self._h = None
+
+    def perform_time_step(self, current_time, time_step, solution,
+                          control=None):
+        control = control or solver_control()
+        p = (ctypes.c_double * 4)()
+        L.check(self.op._lib.dg_irk_integrator_step(
+            self._h, current_time, time_step, self.op._v(solution, "solution"),
+            ctypes.byref(control), p))
+        self.project_sol = np.array(p)
+        return control
+
+
+class DiagonallyImplicitRungeKuttaIntegrator:
+    """advection_solver_implicit_real.cc:1561-1748 (1, 3, 4 or 7 stages)."""
+
+    def __init__(self, n_stages, operator):
+        self.n_stages = n_stages
+        self.op = operator
+
+    def perform_time_step(self, solution, current_time, time_step,
+                          control=None):
+        control = control or solver_control(max_iterations=1000)
+        L.check(self.op._lib.dg_dirk_time_step(
+            self.op._h, self.n_stages, current_time, time_step,
+            self.op._v(solution, "solution"), ctypes.byref(control)))
+        return control
+
+
 def irk_coefficients(stages):
     A = (ctypes.c_double * (stages * stages))()
     b = (ctypes.c_double * stages)()

@@ -242,6 +242,19 @@ int dg_irk_solve(dg_operator *op, void *stage_blocks, const void *rhs_blocks,
 int dg_irk_time_step(dg_operator *op, double current_time, double time_step,
                      void *solution, dg_solver_control *control);
 
+/* Radau IIA integrator that keeps the last four stage solutions and starts
+ * every solve from their least-squares extrapolation (fused Gram matrix,
+ * Cholesky, refresh when the initial residual doubles or after 20 steps)
+ *                                   run() advection_solver_irk_real.cc:2525-2773 */
+typedef struct dg_irk_integrator dg_irk_integrator;
+int dg_irk_integrator_create(dg_operator *op, dg_irk_integrator **it);
+int dg_irk_integrator_destroy(dg_irk_integrator *it);
+/* one step of size dt from t; `project_sol` (4 values, may be null)
+ * receives the extrapolation weights                                    */
+int dg_irk_integrator_step(dg_irk_integrator *it, double current_time,
+                           double time_step, void *solution,
+                           dg_solver_control *control, double *project_sol);
+
 /* One DIRK step with per-stage (F)GMRES solves    perform_time_step
  *                                                 advection_solver_implicit_real.cc:1653-1731 */
 int dg_dirk_time_step(dg_operator *op, int n_stages, double current_time,

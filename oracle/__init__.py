@@ -20,6 +20,7 @@ class OrProblem(ctypes.Structure):
         ("dim", ctypes.c_int),
         ("degree", ctypes.c_int),
         ("n_components", ctypes.c_int),
+        ("operator_kind", ctypes.c_int),
         ("n_coarse", ctypes.c_int * 3),
         ("refine", ctypes.c_int),
         ("lo", ctypes.c_double * 3),
@@ -85,6 +86,7 @@ def load():
         "or_irk_vmult": (None, [_vp, ctypes.c_double, ctypes.c_double, _dp,
                                 _dp]),
         "or_irk_update_solution": (None, [_vp, ctypes.c_double, _dp, _dp]),
+        "or_burgers_apply": (None, [_vp, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -149,6 +151,11 @@ class Oracle:
     def apply(self, src, t):
         dst = np.zeros_like(src)
         self.lib.or_apply(self.h, _p(src), _p(dst), t)
+        return dst
+
+    def burgers_apply(self, src):
+        dst = np.zeros_like(src)
+        self.lib.or_burgers_apply(self.h, _p(src), _p(dst))
         return dst
 
     def loop(self, src, t):

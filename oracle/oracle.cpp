@@ -597,7 +597,8 @@ namespace
             const double *src, int64_t block, double *dst, CellEval &E)
   {
     const int dim = H.dim, n = H.n, nd = H.nd;
-    const double s = H.P.factor_skew;
+    const double s  = H.P.factor_skew;
+    const double nu = H.P.operator_kind == 1 ? H.P.viscosity : 0.;
     const int    ns = L.n_stages;
     std::vector<CellEval> ev(ns);
     for (int c = 0; c < ns; ++c)
@@ -633,13 +634,14 @@ namespace
             if (!L.implicit)
               {
                 for (int e = 0; e < dim; ++e)
-                  F[e] = (1.0 - s) * a[e] * u;
+                  F[e] = (1.0 - s) * a[e] * u - nu * gu[e];
                 V = -s * adotgu;
               }
             else
               {
+                // advection_diffusion_solver.cc:546-549: volume_flux + nu grad u
                 for (int e = 0; e < dim; ++e)
-                  F[e] = (-1.0 + s) * a[e] * u;
+                  F[e] = (-1.0 + s) * a[e] * u + nu * gu[e];
                 V = s * adotgu;
                 if (ns == 1)
                   V += L.inv_dt * u;
@@ -828,6 +830,300 @@ namespace
         }
   }
 
+  // value and unit-cell gradient of a cell's nodal expansion at an arbitrary
+  // reference point, by direct summation over the Lagrange products
+  void
+  point_eval(const or_handle &H, const double *u, const double *xi,
+             double &val, double *gref)
+  {
+    const int n = H.n, dim = H.dim;
+    double    v1[3][8], d1[3][8];
+    for (int e = 0; e < dim; ++e)
+      for (int i = 0; i < n; ++i)
+        {
+          v1[e][i] = (double)lagr(H.xgl, i, xi[e]);
+          d1[e][i] = (double)lagr_der(H.xgl, i, xi[e]);
+        }
+    val = 0;
+    for (int e = 0; e < 3; ++e)
+      gref[e] = 0;
+    for (int idx = 0; idx < H.nd; ++idx)
+      {
+        const int ii[3] = {idx % n, (idx / n) % n, dim == 3 ? idx / (n * n) : 0};
+        double    prod  = 1;
+        for (int e = 0; e < dim; ++e)
+          prod *= v1[e][ii[e]];
+        val += prod * u[idx];
+        for (int g = 0; g < dim; ++g)
+          {
+            double pg = 1;
+            for (int e = 0; e < dim; ++e)
+              pg *= (e == g ? d1[e][ii[e]] : v1[e][ii[e]]);
+            gref[g] += pg * u[idx];
+          }
+      }
+  }
+
+  // dst[i] += value * phi_i(xi) + dn * (n . J^-T grad_ref phi_i)(xi)
+  void
+  point_integrate(const or_handle &H, const double *xi, const double Ji[3][3],
+                  const double *normal, double value, double dn, double *dst)
+  {
+    const int n = H.n, dim = H.dim;
+    double    v1[3][8], d1[3][8];
+    for (int e = 0; e < dim; ++e)
+      for (int i = 0; i < n; ++i)
+        {
+          v1[e][i] = (double)lagr(H.xgl, i, xi[e]);
+          d1[e][i] = (double)lagr_der(H.xgl, i, xi[e]);
+        }
+    for (int idx = 0; idx < H.nd; ++idx)
+      {
+        const int ii[3] = {idx % n, (idx / n) % n, dim == 3 ? idx / (n * n) : 0};
+        double    prod  = 1;
+        for (int e = 0; e < dim; ++e)
+          prod *= v1[e][ii[e]];
+        double gref[3] = {0, 0, 0};
+        for (int g = 0; g < dim; ++g)
+          {
+            double pg = 1;
+            for (int e = 0; e < dim; ++e)
+              pg *= (e == g ? d1[e][ii[e]] : v1[e][ii[e]]);
+            gref[g] = pg;
+          }
+        double ndg = 0;
+        for (int e = 0; e < dim; ++e)
+          {
+            double ge = 0;
+            for (int g = 0; g < dim; ++g)
+              ge += Ji[g][e] * gref[g];
+            ndg += normal[e] * ge;
+          }
+        dst[idx] += value * prod + dn * ndg;
+      }
+  }
+
+  // dst[i] += sum_d grad_ref phi_i(xi)[d] g[d]
+  void
+  point_integrate_grad(const or_handle &H, const double *xi, const double *g,
+                       double *dst)
+  {
+    const int n = H.n, dim = H.dim;
+    double    v1[3][8], d1[3][8];
+    for (int e = 0; e < dim; ++e)
+      for (int i = 0; i < n; ++i)
+        {
+          v1[e][i] = (double)lagr(H.xgl, i, xi[e]);
+          d1[e][i] = (double)lagr_der(H.xgl, i, xi[e]);
+        }
+    for (int idx = 0; idx < H.nd; ++idx)
+      {
+        const int ii[3] = {idx % n, (idx / n) % n, dim == 3 ? idx / (n * n) : 0};
+        double    s     = 0;
+        for (int gd = 0; gd < dim; ++gd)
+          {
+            double pg = 1;
+            for (int e = 0; e < dim; ++e)
+              pg *= (e == gd ? d1[e][ii[e]] : v1[e][ii[e]]);
+            s += pg * g[gd];
+          }
+        dst[idx] += s;
+      }
+  }
+
+  // Burgers loop (burgers_solver.cc:259-341): cell flux u (x) u and Rusanov
+  // faces, all with QGauss(k(k+1)/2), explicit signs; dst = -L(u)
+  void
+  burgers_loop(const or_handle &H, const double *src, double *dst)
+  {
+    const int dim = H.dim, nd = H.nd, nc = dim;
+    const int m   = H.P.degree * (H.P.degree + 1) / 2;
+    std::vector<ld> gx, gw;
+    gauss(m, gx, gw);
+    std::fill(dst, dst + H.n_cells * nc * nd, 0.);
+    const int mp = dim == 3 ? m * m * m : m * m;
+#pragma omp parallel for schedule(static)
+    for (int64_t cell = 0; cell < H.n_cells; ++cell)
+      for (int q = 0; q < mp; ++q)
+        {
+          const int qi[3] = {q % m, (q / m) % m, dim == 3 ? q / (m * m) : 0};
+          double    xi[3] = {(double)gx[qi[0]], (double)gx[qi[1]],
+                             dim == 3 ? (double)gx[qi[2]] : 0.};
+          double    w = (double)(gw[qi[0]] * gw[qi[1]]);
+          if (dim == 3)
+            w *= (double)gw[qi[2]];
+          double x[3], J[3][3], Ji[3][3];
+          map_point(H, cell, xi, x, J);
+          const double det = inverse_jacobian(dim, J, Ji);
+          double       u[3], gr[3];
+          for (int c = 0; c < nc; ++c)
+            point_eval(H, src + (cell * nc + c) * nd, xi, u[c], gr);
+          for (int c = 0; c < nc; ++c)
+            {
+              double g[3];
+              for (int d = 0; d < dim; ++d)
+                {
+                  double s = 0;
+                  for (int e = 0; e < dim; ++e)
+                    s += Ji[d][e] * u[c] * u[e];
+                  g[d] = s * det * w;
+                }
+              point_integrate_grad(H, xi, g, dst + (cell * nc + c) * nd);
+            }
+        }
+    const int mf = dim == 3 ? m * m : m;
+    for (int d = 0; d < dim; ++d)
+      for (int pass = 0; pass < 3; ++pass)
+        {
+#pragma omp parallel for schedule(static)
+          for (int64_t cell = 0; cell < H.n_cells; ++cell)
+            {
+              const int id = H.ijk[3 * cell + d];
+              const int mypass =
+                (H.N[d] % 2 == 1 && id == H.N[d] - 1) ? 2 : id % 2;
+              if (mypass != pass)
+                continue;
+              const int64_t nb = H.nbr[cell * 2 * dim + 2 * d + 1];
+              const int     t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+              for (int fq = 0; fq < mf; ++fq)
+                {
+                  const int a = fq % m, b = dim == 3 ? fq / m : 0;
+                  double    xm[3] = {0, 0, 0}, xp[3] = {0, 0, 0};
+                  xm[d]  = 1;
+                  xp[d]  = 0;
+                  xm[t1] = xp[t1] = (double)gx[a];
+                  double w = (double)gw[a];
+                  if (dim == 3)
+                    {
+                      xm[t2] = xp[t2] = (double)gx[b];
+                      w *= (double)gw[b];
+                    }
+                  double x[3], J[3][3];
+                  map_point(H, cell, xm, x, J);
+                  double nv[3] = {0, 0, 0};
+                  if (dim == 3)
+                    {
+                      const int p1 = (d + 1) % 3, p2 = (d + 2) % 3;
+                      nv[0] = J[1][p1] * J[2][p2] - J[2][p1] * J[1][p2];
+                      nv[1] = J[2][p1] * J[0][p2] - J[0][p1] * J[2][p2];
+                      nv[2] = J[0][p1] * J[1][p2] - J[1][p1] * J[0][p2];
+                    }
+                  else
+                    {
+                      const int t = 1 - d;
+                      nv[0]       = d == 0 ? J[1][t] : -J[1][t];
+                      nv[1]       = d == 0 ? -J[0][t] : J[0][t];
+                    }
+                  double len = 0;
+                  for (int e = 0; e < dim; ++e)
+                    len += nv[e] * nv[e];
+                  len = std::sqrt(len);
+                  double nrm[3];
+                  for (int e = 0; e < dim; ++e)
+                    nrm[e] = nv[e] / len;
+                  const double JxW = len * w;
+                  double       um[3], up[3], gr[3];
+                  for (int c = 0; c < nc; ++c)
+                    {
+                      point_eval(H, src + (cell * nc + c) * nd, xm, um[c], gr);
+                      point_eval(H, src + (nb * nc + c) * nd, xp, up[c], gr);
+                    }
+                  double umn = 0, upn = 0;
+                  for (int e = 0; e < dim; ++e)
+                    {
+                      umn += um[e] * nrm[e];
+                      upn += up[e] * nrm[e];
+                    }
+                  const double lam =
+                    H.P.flux_alpha * std::max(std::fabs(umn), std::fabs(upn));
+                  for (int c = 0; c < nc; ++c)
+                    {
+                      const double flux =
+                        0.5 * ((umn * um[c] + upn * up[c]) +
+                               lam * (um[c] - up[c]));
+                      double Jd[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                      point_integrate(H, xm, Jd, nrm, -flux * JxW, 0.,
+                                      dst + (cell * nc + c) * nd);
+                      point_integrate(H, xp, Jd, nrm, flux * JxW, 0.,
+                                      dst + (nb * nc + c) * nd);
+                    }
+                }
+            }
+        }
+  }
+
+  // SIPG face terms of advection_diffusion_solver.cc:569-613 (interior
+  // faces, the minus side's normal on both sides)
+  void
+  sipg_face_terms(const or_handle &H, const LoopSpec &L, const double *src,
+                  double *dst)
+  {
+    const int    dim = H.dim, n = H.n, nd = H.nd;
+    const int    nf  = dim == 3 ? n * n : n;
+    const double nu  = H.P.viscosity;
+    const double sg  = L.implicit ? 1. : -1.;
+    for (int d = 0; d < dim; ++d)
+      for (int pass = 0; pass < 3; ++pass)
+        {
+#pragma omp parallel for schedule(static)
+          for (int64_t cell = 0; cell < H.n_cells; ++cell)
+            {
+              const int id = H.ijk[3 * cell + d];
+              const int mypass =
+                (H.N[d] % 2 == 1 && id == H.N[d] - 1) ? 2 : id % 2;
+              if (mypass != pass)
+                continue;
+              const int64_t nb = H.nbr[cell * 2 * dim + 2 * d + 1];
+              if (nb < 0)
+                continue;
+              const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+              for (int fq = 0; fq < nf; ++fq)
+                {
+                  int    tq[2] = {fq % n, dim == 3 ? fq / n : 0};
+                  double x[3], normal[3], JxW;
+                  face_geometry(H, cell, d, 1, tq, x, normal, JxW);
+                  double xm[3] = {0, 0, 0}, xp[3] = {0, 0, 0};
+                  xm[d] = 1;
+                  xp[d] = 0;
+                  xm[t1] = xp[t1] = H.xq[tq[0]];
+                  if (dim == 3)
+                    xm[t2] = xp[t2] = H.xq[tq[1]];
+                  double Jm[3][3], Jp[3][3], Jim[3][3], Jip[3][3], xx[3];
+                  map_point(H, cell, xm, xx, Jm);
+                  map_point(H, nb, xp, xx, Jp);
+                  inverse_jacobian(dim, Jm, Jim);
+                  inverse_jacobian(dim, Jp, Jip);
+                  double um, up, gm[3], gp[3];
+                  point_eval(H, src + cell * nd, xm, um, gm);
+                  point_eval(H, src + nb * nd, xp, up, gp);
+                  double dnm = 0, dnp = 0, ihm = 0, ihp = 0;
+                  for (int e = 0; e < dim; ++e)
+                    {
+                      double gme = 0, gpe = 0;
+                      for (int g = 0; g < dim; ++g)
+                        {
+                          gme += Jim[g][e] * gm[g];
+                          gpe += Jip[g][e] * gp[g];
+                        }
+                      dnm += normal[e] * gme;
+                      dnp += normal[e] * gpe;
+                      ihm += Jim[d][e] * normal[e];
+                      ihp += Jip[d][e] * normal[e];
+                    }
+                  const double sigma =
+                    H.P.penalty_factor * 0.5 * (std::fabs(ihm) + std::fabs(ihp));
+                  const double jump    = (um - up) * (-0.5 * nu);
+                  const double avg     = (dnm + dnp) * (-0.5 * nu);
+                  const double viscous = jump * sigma - avg;
+                  point_integrate(H, xm, Jim, normal, -viscous * JxW * sg,
+                                  jump * JxW * sg, dst + cell * nd);
+                  point_integrate(H, xp, Jip, normal, viscous * JxW * sg,
+                                  jump * JxW * sg, dst + nb * nd);
+                }
+            }
+        }
+  }
+
   void
   run_loop(const or_handle &H, const LoopSpec &L, const double *src,
            int64_t block, double *dst, double t_bc)
@@ -841,6 +1137,8 @@ namespace
         cell_term(H, L, cell, src, block, dst, E);
     }
     face_terms(H, L, src, block, dst, t_bc);
+    if (H.P.operator_kind == 1)
+      sipg_face_terms(H, L, src, dst);
   }
 
   void
@@ -1388,6 +1686,31 @@ or_irk_update_solution(or_handle *h, double dt, double *solution,
           my += b[cc] * sum;
         }
       solution[i] += dt * my;
+    }
+}
+
+} // extern "C"
+
+extern "C" {
+
+void
+or_burgers_apply(or_handle *h, const double *src, double *dst)
+{
+  // BurgersOperation::apply: loop then inverse mass per component
+  // (burgers_solver.cc:386-410)
+  const int64_t nc = h->P.n_components, nd = h->nd;
+  std::vector<double> tmp(h->n_cells * nc * nd);
+  burgers_loop(*h, src, tmp.data());
+  std::vector<double> a(h->n_cells * nd), b(h->n_cells * nd);
+  for (int64_t c = 0; c < nc; ++c)
+    {
+      for (int64_t cell = 0; cell < h->n_cells; ++cell)
+        std::copy(tmp.begin() + (cell * nc + c) * nd,
+                  tmp.begin() + (cell * nc + c + 1) * nd, a.begin() + cell * nd);
+      inverse_mass_cells(*h, a.data(), b.data());
+      for (int64_t cell = 0; cell < h->n_cells; ++cell)
+        std::copy(b.begin() + cell * nd, b.begin() + (cell + 1) * nd,
+                  dst + (cell * nc + c) * nd);
     }
 }
 

@@ -16,6 +16,7 @@ extern "C" {
 typedef struct or_problem
 {
   int      dim, degree, n_components;
+  int      operator_kind; /* 0 advection, 1 advection + SIPG, 2 Burgers */
   int      n_coarse[3], refine;
   double   lo[3], hi[3];
   int      periodic[3];
@@ -77,6 +78,9 @@ void or_irk_compute_rhs(or_handle *h, double t, double dt, const double *src,
                         double *dst);
 void or_irk_vmult(or_handle *h, double t, double dt, const double *src,
                   double *dst);
+/* Burgers right-hand side M^-1(-L(u)) (burgers_solver.cc:259-410)       */
+void or_burgers_apply(or_handle *h, const double *src, double *dst);
+
 void or_irk_update_solution(or_handle *h, double dt, double *solution,
                             const double *stage);
 

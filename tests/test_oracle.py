@@ -255,3 +255,72 @@ def test_lsrk_order_of_accuracy():
     e2 = np.linalg.norm(run(16) - ref)
     rate = np.log2(e1 / e2)
     assert 3.7 < rate < 4.5, (e1, e2, rate)
+
+
+def test_sipg_symmetric_and_coercive():
+    """Pure diffusion (a = 0), positive operator: SIPG is symmetric and,
+    with the reference's penalty 2k(k+1), coercive."""
+    k = 3
+    c = Cfg(dim=3, degree=k, refine=1, velocity=[0, 0, 0], operator_kind=1,
+            viscosity=0.7, penalty_factor=2 * k * (k + 1))
+    o = c.oracle_handle()
+    rng = np.random.default_rng(9)
+    u, v = rng.standard_normal((2, o.n_dofs))
+    Lu, Lv = o.vmult(u, 0.0, 0.0), o.vmult(v, 0.0, 0.0)
+    assert abs(v @ Lu - u @ Lv) < 1e-12 * abs(v @ Lu)
+    assert u @ Lu > 0 and v @ Lv > 0
+
+
+def test_sipg_closed_form_decay():
+    """Explicit advection-diffusion right-hand side of the product of sines
+    approaches -a.grad u + nu lap u (spectral accuracy in k)."""
+    nu, k = 0.05, 6
+    a = [1.0, 0.5, 0.25]
+    c = Cfg(dim=3, degree=k, refine=1, velocity=a, operator_kind=1,
+            viscosity=nu, penalty_factor=(k + 1) ** 2, solution_kind=2,
+            solution_param=[2 * np.pi, nu])
+    o = c.oracle_handle()
+    u = o.project(0.0)
+    du = o.apply(u, 0.0)
+    # exact d/dt of prod sin(2 pi (x_d - a_d t)) exp(-3 nu (2 pi)^2 t) at t=0
+    # is projected with the same interpolation as u
+    h = 1e-5
+    ddt = (o.project(h) - o.project(-h)) / (2 * h)
+    assert rel_diff(du, ddt) < 2e-3
+
+
+def _burgers(dim=3, k=3, refine=1, **kw):
+    return Cfg(dim=dim, degree=k, n_components=dim, operator_kind=2,
+               refine=refine, lo=[-1, -1, -1], hi=[1, 1, 1],
+               solution_kind=6, **kw)
+
+
+def test_burgers_constant_state_is_steady():
+    o = _burgers(perturbation=0.1).oracle_handle()
+    u = np.zeros(o.n_dofs * 3)
+    nd = 64
+    for c, val in enumerate((0.7, -0.3, 1.1)):
+        u.reshape(-1, 3, nd)[:, c, :] = val
+    du = o.burgers_apply(u)
+    assert np.max(np.abs(du)) < 1e-12
+
+
+def test_burgers_smooth_right_hand_side():
+    """u = (-sin(pi x), 0, 0): du/dt = -d/dx(u_0 u_0) = -pi sin(2 pi x)."""
+    o = _burgers(k=4, refine=2).oracle_handle()
+    u = o.project(0.0)
+    du = o.burgers_apply(u).reshape(-1, 3, 125)
+    pts = nodal_points(o.cell_vertices(), 5, 3)
+    exact = -np.pi * np.sin(2 * np.pi * pts[..., 0])
+    assert rel_diff(du[:, 0, :], exact) < 2e-2
+    assert np.max(np.abs(du[:, 1:, :])) < 1e-12
+
+
+def test_burgers_conserves_momentum():
+    o = _burgers(perturbation=0.1).oracle_handle()
+    u = np.random.default_rng(3).standard_normal(o.n_dofs * 3)
+    du = o.burgers_apply(u).reshape(-1, 3, 64)
+    for c in range(3):
+        comp = np.ascontiguousarray(du[:, c, :]).ravel()
+        m = o.mass(comp)
+        assert abs(m.sum()) < 1e-11 * np.abs(m).sum()
