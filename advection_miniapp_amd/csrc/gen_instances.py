@@ -29,11 +29,9 @@ def op_variants():
             (2, 3), range(2, 9), (0, 1), (0, 1)):
         v.append((dim, n, 1, fp32, cart, 1, 0, FIN_INTEGRATE, EPI_STORE))
     # stage-coupled IRK operator
-    for dim, n, nc, fp32 in itertools.product(
-            (2, 3), range(2, 8), (2, 3, 4), (0, 1)):
-        v.append((dim, n, nc, fp32, 1, 1, 0, FIN_INTEGRATE, EPI_STORE))
-        if dim == 3 and n == 5:
-            v.append((dim, n, nc, fp32, 0, 1, 0, FIN_INTEGRATE, EPI_STORE))
+    for dim, n, nc, fp32, cart in itertools.product(
+            (2, 3), range(2, 8), (2, 3, 4), (0, 1), (0, 1)):
+        v.append((dim, n, nc, fp32, cart, 1, 0, FIN_INTEGRATE, EPI_STORE))
     # advection + SIPG diffusion, axis-aligned: explicit (apply, LSRK stage)
     # and implicit operator, fp64 and fp32
     for dim, n, fp32 in itertools.product((2, 3), range(2, 9), (0, 1)):

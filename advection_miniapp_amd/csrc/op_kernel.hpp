@@ -32,6 +32,7 @@
 #include "analytic.hpp"
 #include "dev_basis.hpp"
 #include "dg_common.hpp"
+#include "line_geometry.hpp"
 
 namespace dgops
 {
@@ -186,81 +187,6 @@ namespace dgops
 
   // ---- geometry ------------------------------------------------------------
 
-  // Multilinear map of a cell from its 2^dim corners X[v*dim+e]:
-  // position x and Jacobian J[e][d] = dx_e / dxi_d at reference point xi.
-  template <int dim>
-  __device__ __forceinline__ void
-  multilinear_map(const double *X,
-                  const double (&xi)[3],
-                  double (&x)[3],
-                  double (&J)[3][3])
-  {
-    if constexpr (dim == 3)
-      {
-        const double a = xi[0], b = xi[1], c = xi[2];
-        const double ma = 1 - a, mb = 1 - b, mc = 1 - c;
-        for (int e = 0; e < 3; ++e)
-          {
-            const double x0 = X[0 * 3 + e], x1 = X[1 * 3 + e],
-                         x2 = X[2 * 3 + e], x3 = X[3 * 3 + e],
-                         x4 = X[4 * 3 + e], x5 = X[5 * 3 + e],
-                         x6 = X[6 * 3 + e], x7 = X[7 * 3 + e];
-            const double e01 = ma * x0 + a * x1, e23 = ma * x2 + a * x3,
-                         e45 = ma * x4 + a * x5, e67 = ma * x6 + a * x7;
-            const double f0 = mb * e01 + b * e23, f1 = mb * e45 + b * e67;
-            x[e] = mc * f0 + c * f1;
-            J[e][0] = mc * (mb * (x1 - x0) + b * (x3 - x2)) +
-                      c * (mb * (x5 - x4) + b * (x7 - x6));
-            J[e][1] = mc * (e23 - e01) + c * (e67 - e45);
-            J[e][2] = f1 - f0;
-          }
-      }
-    else
-      {
-        const double a = xi[0], b = xi[1];
-        const double ma = 1 - a, mb = 1 - b;
-        for (int e = 0; e < 2; ++e)
-          {
-            const double x0 = X[0 * 2 + e], x1 = X[1 * 2 + e],
-                         x2 = X[2 * 2 + e], x3 = X[3 * 2 + e];
-            const double e01 = ma * x0 + a * x1, e23 = ma * x2 + a * x3;
-            x[e]    = mb * e01 + b * e23;
-            J[e][0] = mb * (x1 - x0) + b * (x3 - x2);
-            J[e][1] = e23 - e01;
-          }
-        x[2] = 0;
-      }
-  }
-
-  // rows of the adjugate: adj[d][e] = det(J) * (J^-1)[d][e]
-  template <int dim>
-  __device__ __forceinline__ double
-  adjugate(const double (&J)[3][3], double (&adj)[3][3])
-  {
-    if constexpr (dim == 3)
-      {
-        // columns c_d = J[.][d]; adj row 0 = c1 x c2, row 1 = c2 x c0,
-        // row 2 = c0 x c1
-        for (int d = 0; d < 3; ++d)
-          {
-            const int d1 = (d + 1) % 3, d2 = (d + 2) % 3;
-            adj[d][0] = J[1][d1] * J[2][d2] - J[2][d1] * J[1][d2];
-            adj[d][1] = J[2][d1] * J[0][d2] - J[0][d1] * J[2][d2];
-            adj[d][2] = J[0][d1] * J[1][d2] - J[1][d1] * J[0][d2];
-          }
-        return J[0][0] * adj[0][0] + J[1][0] * adj[0][1] +
-               J[2][0] * adj[0][2];
-      }
-    else
-      {
-        adj[0][0] = J[1][1];
-        adj[0][1] = -J[0][1];
-        adj[1][0] = -J[1][0];
-        adj[1][1] = J[0][0];
-        return J[0][0] * J[1][1] - J[0][1] * J[1][0];
-      }
-  }
-
   // Per quadrature point data of the cell term: cv[d] = w * (adj(J) a)_d
   // (the reference-space transport speed times JxW, which is what
   // advection_solver_implicit_real.cc:1126-1128 and :998-1007 fold
@@ -390,14 +316,6 @@ namespace dgops
       {
         const int f = 2 * dir + s;
         const T(&l)[max_n] = s ? B.l1 : B.l0;
-        if (compute_geometry)
-          {
-            double an = 0, x[3];
-            if (active)
-              face_point_data<dim, cartesian>(P, X, Bd->xq, Bd->w, dir, s, q,
-                                              an, x);
-            fan[f] = (T)an;
-          }
         T um = l[0] * uq[0];
 #pragma unroll
         for (int i = 1; i < n; ++i)
@@ -537,6 +455,7 @@ namespace dgops
     __shared__ T      s_sk0[skew ? CPB * STR : 1];
     __shared__ T      s_sk1[(skew && dim == 3) ? CPB * STR : 1];
     __shared__ T      s_face[dim == 3 ? CPB * FSTR : 1];
+    __shared__ T      s_fan[CPB * 2 * (dim - 1) * nl];
     __shared__ double s_X[CPB * NX];
 
     const DevBasis<T> &B    = *Bp;
@@ -554,6 +473,7 @@ namespace dgops
     T      *sk0_l = skew ? s_sk0 + lcs * STR : s_sk0;
     T      *sk1_l = (skew && dim == 3) ? s_sk1 + lcs * STR : s_sk1;
     T      *f_l   = dim == 3 ? s_face + lcs * FSTR : s_face;
+    T      *fan_l = s_fan + lcs * 2 * (dim - 1) * nl;
     double *X     = s_X + lcs * NX;
 
     if (active)
@@ -723,22 +643,45 @@ namespace dgops
         __syncthreads();
       }
 
-    // ---- geometry and velocity at this thread's LAST-line points
+    // ---- geometry and velocity: this thread's LAST-line points and the
+    // faces its line ends on; faces normal to the other directions are
+    // evaluated along LAST-direction lines by up to one task per thread and
+    // handed over through LDS
     T cv[3][n], jxw[n];
-#pragma unroll
-    for (int i = 0; i < n; ++i)
-      {
-        int    q[3];
-        double cvd[3] = {0, 0, 0}, J = 1.;
-        L::template idx<LAST>(p, i, q);
-        if (active)
-          cell_point_data<dim, cartesian>(P, X, Bd->xq, Bd->w, q, cvd, J);
-        cv[0][i] = (T)cvd[0];
-        cv[1][i] = (T)cvd[1];
-        cv[2][i] = (T)cvd[2];
-        jxw[i]   = (T)J;
-      }
     T fan[2 * dim];
+    {
+      int q[3];
+      L::template idx<LAST>(p, 0, q);
+      T fl[2] = {0, 0};
+      if (active)
+        line_cell_data<dim, n, LAST, cartesian, T>(P, X, Bd, q, cv, jxw, fl);
+      else
+        {
+#pragma unroll
+          for (int i = 0; i < n; ++i)
+            {
+              cv[0][i] = cv[1][i] = cv[2][i] = 0;
+              jxw[i]                         = 1;
+            }
+        }
+      fan[2 * LAST]     = fl[0];
+      fan[2 * LAST + 1] = fl[1];
+      constexpr int ntask = dim == 3 ? 4 * n : 2;
+      if (active)
+        for (int task = p; task < ntask; task += nl)
+          {
+            const int d = dim == 3 ? task / (2 * n) : 0;
+            const int s = dim == 3 ? (task / n) % 2 : task;
+            const int j = dim == 3 ? task % n : 0;
+            face_line_data<dim, n, LAST, cartesian, T>(
+              P, X, Bd, d, s, j, fan_l + (2 * d + s) * nl + j * n);
+          }
+      __syncthreads();
+      const int fidx = dim == 3 ? (p % n) * n + p / n : p;
+#pragma unroll
+      for (int f = 0; f < 2 * LAST; ++f)
+        fan[f] = fan_l[f * nl + fidx];
+    }
 
 #pragma unroll
     for (int c = 0; c < NC; ++c)

@@ -381,7 +381,7 @@ def test_irk_integrator_projection(gpu_lib):
     u_b = u_a.clone()
     integ = ImplicitRungeKuttaIntegrator(op_a)
     dt, t = 0.02, 0.0
-    res_proj, res_zero = [], []
+    res_proj, res_zero, log = [], [], []
     for step in range(8):
         ctl = control(rel_tolerance=1e-13, restart=30)
         integ.perform_time_step(t, dt, u_a, ctl)
@@ -391,7 +391,11 @@ def test_irk_integrator_projection(gpu_lib):
                                            op_b._v(u_b, "u"),
                                            ctypes.byref(ctl2)))
         res_zero.append(ctl2.initial_residual)
+        log.append((step, rel_diff(host(u_a), host(u_b)), ctl.iterations,
+                    ctl.initial_residual, ctl.final_residual,
+                    ctl2.iterations, ctl2.initial_residual,
+                    ctl2.final_residual, tuple(integ.project_sol)))
         t += dt
-    assert rel_diff(host(u_a), host(u_b)) < 1e-10
+    assert rel_diff(host(u_a), host(u_b)) < 1e-10, log
     assert all(p < 0.1 * z for p, z in zip(res_proj[3:], res_zero[3:])), \
         (res_proj, res_zero)

@@ -48,7 +48,11 @@ def test_sipg_fp32_error(gpu_lib, k):
     dst = op.initialize_dof_vector()
     op.apply(dev(u, torch.float32), dst, 0.0)
     err = rel_diff(host(dst), o.apply(u, 0.0))
-    assert err < 2e-5, err
+    # FP32 rounding of the input alone, pushed through the FP64 operator:
+    # the second-derivative terms amplify it by ~k^4 / h^2 / |L u|
+    u32 = u.astype(np.float32).astype(np.float64)
+    floor = rel_diff(o.apply(u32, 0.0), o.apply(u, 0.0))
+    assert err < max(20 * floor, 2e-5), (err, floor)
 
 
 def test_sipg_lsrk_stage_matches_oracle(gpu_lib):
