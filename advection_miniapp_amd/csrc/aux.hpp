@@ -29,6 +29,11 @@ namespace dgops
                       void *y);
   void vec_multi_dot(Operator &op, int p, int64_t n, int m, const void *x,
                      int64_t x_stride, const void *y, double *result);
+  // y += sum_j alpha_j x_j, then result[j] = <x_j, y> (j < m) and
+  // result[m] = <y, y> of the updated y, in one pass for m <= 32
+  void vec_multi_axpy_dot(Operator &op, int p, int64_t n, int m,
+                          const double *alpha, const void *x,
+                          int64_t x_stride, void *y, double *result);
   void vec_copy_convert(Operator &op, int64_t n, int ps, const void *src,
                         int pd, void *dst);
   // one pass over four vectors and a right-hand side: the lower triangle of

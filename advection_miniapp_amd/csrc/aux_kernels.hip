@@ -454,6 +454,64 @@ namespace dgops
         }
     }
 
+    // y += sum_j c_j x_j, then <x_j, y> (j < M) and <y, y> of the updated
+    // y in the same pass: the subtraction of one Gram-Schmidt sweep fused
+    // with the projections of the next
+    constexpr int max_fused = 32;
+    struct FusedCoeffs
+    {
+      double v[max_fused];
+    };
+
+    template <typename T, int M>
+    __global__ void __launch_bounds__(vec_threads)
+    multi_axpy_dot_kernel(int64_t n, const FusedCoeffs c,
+                          const T *__restrict__ x, int64_t xs, T *y,
+                          double *partials)
+    {
+      __shared__ double red[M + 1][vec_threads / 64];
+      double            s[M + 1];
+#pragma unroll
+      for (int j = 0; j <= M; ++j)
+        s[j] = 0;
+      for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+           i += (int64_t)gridDim.x * blockDim.x)
+        {
+          T xv[M];
+          T yi = y[i];
+#pragma unroll
+          for (int j = 0; j < M; ++j)
+            {
+              xv[j] = x[j * xs + i];
+              yi    = fma((T)c.v[j], xv[j], yi);
+            }
+          y[i] = yi;
+          const double yd = (double)yi;
+#pragma unroll
+          for (int j = 0; j < M; ++j)
+            s[j] = fma((double)xv[j], yd, s[j]);
+          s[M] = fma(yd, yd, s[M]);
+        }
+      const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+#pragma unroll
+      for (int j = 0; j <= M; ++j)
+        {
+          double v = s[j];
+          for (int off = 32; off > 0; off >>= 1)
+            v += __shfl_down(v, off, 64);
+          if (lane == 0)
+            red[j][wave] = v;
+        }
+      __syncthreads();
+      if (threadIdx.x <= M)
+        {
+          double v = 0;
+          for (int w = 0; w < vec_threads / 64; ++w)
+            v += red[threadIdx.x][w];
+          partials[blockIdx.x * (M + 1) + threadIdx.x] = v;
+        }
+    }
+
     __global__ void
     gram4_kernel(int64_t n, const double *__restrict__ v0,
                  const double *__restrict__ v1, const double *__restrict__ v2,
@@ -567,6 +625,48 @@ namespace dgops
                 "copy result");
       hip_check(hipStreamSynchronize(op.stream), "sync");
       for (int j = 0; j < m; ++j)
+        result[j] = host[j];
+    }
+
+    template <typename T>
+    void
+    multi_axpy_dot_t(Operator &op, int64_t n, int m, const double *alpha,
+                     const T *x, int64_t xs, T *y, double *result)
+    {
+      const unsigned nb = std::min<unsigned>(grid_for(n), 1024);
+      op.d_partials.alloc(sizeof(double) * nb * (max_fused + 1));
+      op.d_result.alloc(sizeof(double) * 64);
+      FusedCoeffs c = {};
+      for (int j = 0; j < m; ++j)
+        c.v[j] = alpha[j];
+      double *part = op.d_partials.as<double>();
+#define DG_MAD(M)                                                           \
+  case M:                                                                   \
+    hipLaunchKernelGGL((multi_axpy_dot_kernel<T, M>), dim3(nb),             \
+                       dim3(vec_threads), 0, op.stream, n, c, x, xs, y,     \
+                       part);                                               \
+    break;
+      switch (m)
+        {
+          DG_MAD(1) DG_MAD(2) DG_MAD(3) DG_MAD(4) DG_MAD(5) DG_MAD(6)
+          DG_MAD(7) DG_MAD(8) DG_MAD(9) DG_MAD(10) DG_MAD(11) DG_MAD(12)
+          DG_MAD(13) DG_MAD(14) DG_MAD(15) DG_MAD(16) DG_MAD(17) DG_MAD(18)
+          DG_MAD(19) DG_MAD(20) DG_MAD(21) DG_MAD(22) DG_MAD(23) DG_MAD(24)
+          DG_MAD(25) DG_MAD(26) DG_MAD(27) DG_MAD(28) DG_MAD(29) DG_MAD(30)
+          DG_MAD(31) DG_MAD(32)
+        }
+#undef DG_MAD
+      hip_check(hipGetLastError(), "multi_axpy_dot");
+      hipLaunchKernelGGL(reduce_columns, dim3(1), dim3(1024), 0, op.stream,
+                         part, (int64_t)nb, m + 1, op.d_result.as<double>());
+      hip_check(hipGetLastError(), "reduce");
+      std::vector<double> host(m + 1);
+      hip_check(hipMemcpyAsync(host.data(), op.d_result.ptr,
+                               sizeof(double) * (m + 1),
+                               hipMemcpyDeviceToHost, op.stream),
+                "copy result");
+      hip_check(hipStreamSynchronize(op.stream), "sync");
+      for (int j = 0; j <= m; ++j)
         result[j] = host[j];
     }
 
@@ -771,6 +871,35 @@ namespace dgops
     else
       multi_axpy_t<double>(op, n, m, alpha, (const double *)x, xs,
                            (double *)y);
+  }
+
+  void
+  vec_multi_axpy_dot(Operator &op, int p, int64_t n, int m,
+                     const double *alpha, const void *x, int64_t xs, void *y,
+                     double *result)
+  {
+    if (m <= 0)
+      return;
+    if (m > max_fused)
+      {
+        // more vectors than one fused pass holds: two passes
+        vec_multi_axpy(op, p, n, m, alpha, x, xs, y);
+        vec_multi_dot(op, p, n, m, x, xs, y, result);
+        vec_multi_dot(op, p, n, 1, y, n, y, result + m);
+        return;
+      }
+    if (n <= 0)
+      {
+        for (int j = 0; j <= m; ++j)
+          result[j] = 0;
+        return;
+      }
+    if (p == DG_FP32)
+      multi_axpy_dot_t<float>(op, n, m, alpha, (const float *)x, xs,
+                              (float *)y, result);
+    else
+      multi_axpy_dot_t<double>(op, n, m, alpha, (const double *)x, xs,
+                               (double *)y, result);
   }
 
   void

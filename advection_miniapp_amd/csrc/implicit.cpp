@@ -393,8 +393,11 @@ namespace dgops
       bool   converged;
     };
 
-    // Arnoldi with classical Gram-Schmidt: one fused pass computes
-    // V^T w and |w|^2; a second pass runs when cancellation exceeds 90%.
+    // Arnoldi with classical Gram-Schmidt: one pass computes V^T w and
+    // |w|^2. When more than half of |w|^2 cancels, a second sweep runs
+    // (Kahan-Parlett "twice is enough"; one CGS sweep alone loses
+    // orthogonality like eps |r0|^2 / |r_k|^2 and stalls GMRES near a
+    // 1e-8 reduction); its projections come out of the subtraction pass.
     KrylovResult
     gmres(Operator &op, const LinOp &L, void *x, const void *b, int restart,
           int max_it, double abs_tol, bool flexible, size_t slot0,
@@ -455,11 +458,14 @@ namespace dgops
               std::vector<double> neg(j + 1);
               for (int k = 0; k <= j; ++k)
                 neg[k] = -h[k];
-              vec_multi_axpy(op, L.prec, n, j + 1, neg.data(), V, n, w);
               double hn2 = ww - hh;
-              if (hn2 < 0.01 * ww)
+              if (hn2 >= 0.5 * ww)
+                vec_multi_axpy(op, L.prec, n, j + 1, neg.data(), V, n, w);
+              else
                 {
-                  vec_multi_dot(op, L.prec, n, j + 2, V, n, w, h2.data());
+                  // w -= V h with h2 = V^T w, |w|^2 of the result
+                  vec_multi_axpy_dot(op, L.prec, n, j + 1, neg.data(), V, n,
+                                     w, h2.data());
                   for (int k = 0; k <= j; ++k)
                     {
                       neg[k] = -h2[k];

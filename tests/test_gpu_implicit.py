@@ -251,6 +251,34 @@ def test_irk_solve_reaches_tolerance(gpu_lib, variant):
     assert ctl.iterations > 0
 
 
+@pytest.mark.parametrize("restart", [10, 30])
+def test_irk_solve_long_cycle_true_residual(gpu_lib, restart):
+    """A long Arnoldi cycle down to a 1e-13 reduction: the true residual
+    follows the recurrence estimate (no loss of orthogonality, no false
+    breakdown)."""
+    s = 3
+    c = Cfg(dim=2, degree=4, refine=3, velocity_kind=3, time_factor=1,
+            final_time=8.0, factor_skew=0.5, irk_stages=s, solution_kind=5)
+    op = AdvectionOperation(c.dg())
+    u = op.initialize_dof_vector()
+    op.project_initial(u, 0.0)
+    op.set_time(0.0, 0.02)
+    rhs = op.initialize_dof_vector(blocks=s)
+    op.compute_rhs(rhs, u)
+    for flexible in (1, 0):
+        z = op.initialize_dof_vector(blocks=s)
+        ctl = control(rel_tolerance=1e-13, restart=restart, flexible=flexible)
+        irk_solve(op, z, rhs, ctl)
+        res = op.initialize_dof_vector(blocks=s)
+        op.vmult_irk(res, z)
+        res -= rhs
+        true_rel = (res.norm() / rhs.norm()).item()
+        assert true_rel < 2e-13, (flexible, true_rel, ctl.iterations,
+                                  ctl.final_residual)
+        assert abs(res.norm().item() - ctl.final_residual) < \
+            0.5 * res.norm().item() + 1e-14 * rhs.norm().item()
+
+
 def test_irk_time_step_matches_dense_cpu_solve(gpu_lib):
     """One Radau IIA(3) step on a small 2D problem against a CPU GMRES
     (scipy) on the oracle's stage operator, both to 1e-13."""
