@@ -21,6 +21,9 @@ namespace dgops
   // (CellwiseInverseMassMatrix::apply, advection_solver_implicit_real.cc:894-909)
   void aux_inverse_mass(const Operator &op, int p, void *vec);
 
+  // geometry tables of the operator kernels in precision p (see OpParams)
+  void aux_geometry_tables(Operator &op, int p);
+
   // vector operations, precision p (DG_FP64 / DG_FP32)
   void vec_axpby(Operator &op, int p, int64_t n, double a, const void *x,
                  double b, void *y);

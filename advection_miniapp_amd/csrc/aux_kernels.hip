@@ -722,6 +722,108 @@ namespace dgops
       return Q;
     }
 
+    // ---- geometry tables of the operator kernels ---------------------------
+
+    template <int dim, int n>
+    __device__ void
+    face_line_point(int d, int p, int (&q)[3])
+    {
+      if (d == 0)
+        Lay<dim, n>::template idx<0>(p, 0, q);
+      else if (d == 1)
+        Lay<dim, n>::template idx<1>(p, 0, q);
+      else
+        Lay<dim, n>::template idx<dim - 1>(p, 0, q);
+    }
+
+    // one block per cell: w adj(J) a, JxW and 1/JxW at the Gauss points
+    // (lexicographic), and w (adj(J) a)_d on the lower face of every
+    // direction, indexed by the line p of direction d that ends on it
+    template <int dim, int n>
+    __global__ void
+    geometry_table_kernel(const OpParams P,
+                          const DevBasis<double> *__restrict__ B,
+                          double *tab_cell, double *tab_face)
+    {
+      constexpr int NV = 1 << dim;
+      constexpr int nl = Lay<dim, n>::nl, nd = Lay<dim, n>::nd;
+      __shared__ double X[NV * dim];
+      const int64_t     cell = blockIdx.x;
+      for (int e = threadIdx.x; e < NV * dim; e += blockDim.x)
+        X[e] = P.cell_vertices[cell * NV * dim + e];
+      __syncthreads();
+      double *tc = tab_cell + cell * (dim + 2) * nd;
+      for (int pt = threadIdx.x; pt < nd; pt += blockDim.x)
+        {
+          const int q[3] = {pt % n, (pt / n) % n,
+                            dim == 3 ? pt / (n * n) : 0};
+          double    cv[3], jxw;
+          cell_point_data<dim, false>(P, X, B->xq, B->w, q, cv, jxw);
+          for (int d = 0; d < dim; ++d)
+            tc[d * nd + pt] = cv[d];
+          tc[dim * nd + pt]       = jxw;
+          tc[(dim + 1) * nd + pt] = 1. / jxw;
+        }
+      for (int t = threadIdx.x; t < dim * nl; t += blockDim.x)
+        {
+          const int d = t / nl, p = t % nl;
+          int       q[3];
+          face_line_point<dim, n>(d, p, q);
+          double an, x[3];
+          face_point_data<dim, false>(P, X, B->xq, B->w, d, 0, q, an, x);
+          tab_face[(cell * dim + d) * nl + p] = -an;
+        }
+    }
+
+    // one block per boundary face: outward a . n dA and the physical point
+    template <int dim, int n>
+    __global__ void
+    boundary_table_kernel(const OpParams P,
+                          const DevBasis<double> *__restrict__ B,
+                          const int *bcell, const int *bface, double *tab)
+    {
+      constexpr int NV = 1 << dim;
+      constexpr int nl = Lay<dim, n>::nl;
+      __shared__ double X[NV * dim];
+      const int64_t     b    = blockIdx.x;
+      const int64_t     cell = bcell[b];
+      const int         f    = bface[b];
+      for (int e = threadIdx.x; e < NV * dim; e += blockDim.x)
+        X[e] = P.cell_vertices[cell * NV * dim + e];
+      __syncthreads();
+      for (int p = threadIdx.x; p < nl; p += blockDim.x)
+        {
+          int q[3];
+          face_line_point<dim, n>(f / 2, p, q);
+          double an, x[3] = {0, 0, 0};
+          face_point_data<dim, false>(P, X, B->xq, B->w, f / 2, f % 2, q, an,
+                                      x);
+          double *r = tab + (b * nl + p) * (1 + dim);
+          r[0]      = an;
+          for (int e = 0; e < dim; ++e)
+            r[1 + e] = x[e];
+        }
+    }
+
+    template <int dim, int n>
+    void
+    launch_geometry_tables(Operator &op, const OpParams &P, const int *bcell,
+                           const int *bface, int64_t n_bnd)
+    {
+      const auto *B = op.d_basis_d.as<DevBasis<double>>();
+      if (op.mesh.n_cells > 0)
+        hipLaunchKernelGGL((geometry_table_kernel<dim, n>),
+                           dim3(op.mesh.n_cells), dim3(aux_threads), 0,
+                           op.stream, P, B, op.d_tab_cell[0].as<double>(),
+                           op.d_tab_face[0].as<double>());
+      hip_check(hipGetLastError(), "geometry tables");
+      if (n_bnd > 0)
+        hipLaunchKernelGGL((boundary_table_kernel<dim, n>), dim3(n_bnd),
+                           dim3(64), 0, op.stream, P, B, bcell, bface,
+                           op.d_tab_bnd.as<double>());
+      hip_check(hipGetLastError(), "boundary tables");
+    }
+
     void
     reduce_to_host(Operator &op, int64_t count, int k, double *out)
     {
@@ -778,6 +880,79 @@ namespace dgops
       hipLaunchKernelGGL((inverse_mass_kernel<2, float>), g, b, 0, op.stream,
                          P, B, op.n, (float *)vec);
     hip_check(hipGetLastError(), "inverse mass");
+  }
+
+  void
+  aux_geometry_tables(Operator &op, int p)
+  {
+    const int pi = p == DG_FP32 ? 1 : 0;
+    if (op.tables_ready[pi])
+      return;
+    const int     dim = op.cfg.dim, n = op.n;
+    const int     nl  = dim == 3 ? n * n : n;
+    const int64_t nc  = op.mesh.n_cells;
+    if (!op.tables_ready[0])
+      {
+        // boundary faces in cell order
+        std::vector<int> bidx(std::max<int64_t>(1, nc * 2 * dim), -1);
+        std::vector<int> bcell, bface;
+        for (int64_t c = 0; c < nc; ++c)
+          for (int f = 0; f < 2 * dim; ++f)
+            if (op.mesh.neighbors[c * 2 * dim + f] < 0)
+              {
+                bidx[c * 2 * dim + f] = (int)bcell.size();
+                bcell.push_back((int)c);
+                bface.push_back(f);
+              }
+        const int64_t nb = (int64_t)bcell.size();
+        op.d_bnd_index.alloc(sizeof(int) * bidx.size());
+        hip_check(hipMemcpy(op.d_bnd_index.ptr, bidx.data(),
+                            sizeof(int) * bidx.size(), hipMemcpyHostToDevice),
+                  "upload boundary index");
+        DevBuf d_bcell, d_bface;
+        d_bcell.alloc(sizeof(int) * std::max<int64_t>(1, nb));
+        d_bface.alloc(sizeof(int) * std::max<int64_t>(1, nb));
+        if (nb > 0)
+          {
+            hip_check(hipMemcpy(d_bcell.ptr, bcell.data(), sizeof(int) * nb,
+                                hipMemcpyHostToDevice),
+                      "upload boundary cells");
+            hip_check(hipMemcpy(d_bface.ptr, bface.data(), sizeof(int) * nb,
+                                hipMemcpyHostToDevice),
+                      "upload boundary faces");
+          }
+        op.d_tab_cell[0].alloc(sizeof(double) * (dim + 2) * op.n_dofs);
+        op.d_tab_face[0].alloc(sizeof(double) * dim * nl *
+                               std::max<int64_t>(1, nc));
+        op.d_tab_bnd.alloc(sizeof(double) * (1 + dim) * nl *
+                           std::max<int64_t>(1, nb));
+        const OpParams P = op.base_params(0, 1);
+#define DG_GT(D, N)                                                         \
+  else if (dim == D && n == N)                                              \
+    launch_geometry_tables<D, N>(op, P, d_bcell.as<int>(), d_bface.as<int>(), \
+                                 nb);
+        if (false)
+          ;
+        DG_GT(2, 2) DG_GT(2, 3) DG_GT(2, 4) DG_GT(2, 5) DG_GT(2, 6)
+        DG_GT(2, 7) DG_GT(2, 8) DG_GT(3, 2) DG_GT(3, 3) DG_GT(3, 4)
+        DG_GT(3, 5) DG_GT(3, 6) DG_GT(3, 7) DG_GT(3, 8)
+#undef DG_GT
+        // the temporaries above are freed at scope end: finish first
+        hip_check(hipStreamSynchronize(op.stream), "geometry tables");
+        op.tables_ready[0] = true;
+      }
+    if (pi == 1)
+      {
+        const int64_t ncell = (dim + 2) * op.n_dofs;
+        const int64_t nface = (int64_t)dim * nl * nc;
+        op.d_tab_cell[1].alloc(sizeof(float) * std::max<int64_t>(1, ncell));
+        op.d_tab_face[1].alloc(sizeof(float) * std::max<int64_t>(1, nface));
+        vec_copy_convert(op, ncell, DG_FP64, op.d_tab_cell[0].ptr, DG_FP32,
+                         op.d_tab_cell[1].ptr);
+        vec_copy_convert(op, nface, DG_FP64, op.d_tab_face[0].ptr, DG_FP32,
+                         op.d_tab_face[1].ptr);
+        op.tables_ready[1] = true;
+      }
   }
 
   void

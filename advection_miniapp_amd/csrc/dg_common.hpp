@@ -47,6 +47,15 @@ namespace dgops
     double        sol_param[8];
     double        viscosity;
     double        penalty;
+    // geometry tables of the tabulated-geometry kernels (scalar type of
+    // the kernel): per cell [dim + 2][nd] = w adj(J) a (dim rows), JxW,
+    // 1/JxW at the Gauss points; per cell [dim][nl] = w (adj(J) a)_d on
+    // the face at xi_d = 0 (normal along +xi_d); per boundary face
+    // [nl][1 + dim] (double) = outward a . n dA and the physical point
+    const void   *tab_cell;
+    const void   *tab_face;
+    const double *tab_bnd;
+    const int    *bnd_index; // [cell][2 dim] boundary face id, -1 inside
   };
 
   // LSRK stage update (advection_solver.cc:728-751)

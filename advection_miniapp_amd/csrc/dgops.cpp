@@ -216,8 +216,23 @@ namespace dgops
       throw Error(DG_ERR_UNSUPPORTED,
                   "advection-diffusion (SIPG) is built for axis-aligned "
                   "meshes (perturbation 0)");
+    // analytic geometry only where it is free (axis-aligned cells and a
+    // constant transport speed, or SIPG); everything else reads the
+    // precomputed geometry tables
+    const bool analytic =
+      diff || (P.cartesian && op.cfg.velocity_kind == DG_VEL_CONSTANT);
+    OpParams Pk = P;
+    if (!analytic)
+      {
+        Operator &m = const_cast<Operator &>(op);
+        aux_geometry_tables(m, fp32 ? DG_FP32 : DG_FP64);
+        Pk.tab_cell  = m.d_tab_cell[fp32 ? 1 : 0].ptr;
+        Pk.tab_face  = m.d_tab_face[fp32 ? 1 : 0].ptr;
+        Pk.tab_bnd   = m.d_tab_bnd.as<double>();
+        Pk.bnd_index = m.d_bnd_index.as<int>();
+      }
     const uint32_t key = op_key(op.cfg.dim, op.n, nc, fp32 ? 1 : 0,
-                                P.cartesian ? 1 : 0, skew ? 1 : 0, diff, fin,
+                                analytic ? 1 : 0, skew ? 1 : 0, diff, fin,
                                 epi);
     auto it = op_registry().find(key);
     if (it == op_registry().end())
@@ -227,10 +242,10 @@ namespace dgops
                     " degree=" + std::to_string(op.n - 1) +
                     " components=" + std::to_string(nc) +
                     (fp32 ? " fp32" : " fp64") +
-                    (P.cartesian ? " cartesian" : " multilinear") +
+                    (analytic ? " analytic geometry" : " tabulated geometry") +
                     (skew ? " skew" : "") + " fin=" + std::to_string(fin) +
                     " epi=" + std::to_string(epi));
-    hip_check(it->second(P, op.basis_ptr(fp32), op.d_basis_d.ptr, io,
+    hip_check(it->second(Pk, op.basis_ptr(fp32), op.d_basis_d.ptr, io,
                          op.stream),
               "operator launch");
   }

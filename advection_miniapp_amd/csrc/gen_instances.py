@@ -9,7 +9,9 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 OUT = os.path.join(HERE, "generated")
 
-# operator variant: (dim, n, nc, fp32, cart, skew, diff, fin, epi)
+# operator variant: (dim, n, nc, fp32, cart, skew, diff, fin, epi); cart 1
+# evaluates the geometry analytically (axis-aligned cells with a constant
+# speed, or SIPG), cart 0 reads the precomputed geometry tables
 FIN_INVMASS, FIN_INTEGRATE = 0, 1
 EPI_STORE, EPI_LSRK = 0, 1
 
@@ -20,9 +22,9 @@ def op_variants():
     for dim, n, cart, epi in itertools.product(
             (2, 3), range(2, 9), (0, 1), (EPI_STORE, EPI_LSRK)):
         v.append((dim, n, 1, 0, cart, 0, 0, FIN_INVMASS, epi))
-    # fp32 explicit right-hand side on axis-aligned meshes
-    for dim, n in itertools.product((2, 3), range(2, 9)):
-        v.append((dim, n, 1, 1, 1, 0, 0, FIN_INVMASS, EPI_STORE))
+    # fp32 explicit right-hand side
+    for dim, n, cart in itertools.product((2, 3), range(2, 9), (0, 1)):
+        v.append((dim, n, 1, 1, cart, 0, 0, FIN_INVMASS, EPI_STORE))
     # implicit single-stage operator (M/dt + L) and IRK right-hand side,
     # skew-symmetric form, fp64 and fp32
     for dim, n, cart, fp32 in itertools.product(

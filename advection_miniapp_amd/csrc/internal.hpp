@@ -108,6 +108,12 @@ namespace dgops
     DevBuf  d_fdm_d, d_fdm_f, d_cell_velocity, d_cell_invdet;
     DevBuf  d_burgers; // over-integration tables of the Burgers operator
 
+    // tabulated geometry of the operator kernels ([0] fp64, [1] fp32):
+    // built on first use, time independent (the velocity's time factor
+    // is applied in the kernel)
+    DevBuf d_tab_cell[2], d_tab_face[2], d_tab_bnd, d_bnd_index;
+    bool   tables_ready[2] = {false, false};
+
     // scratch for reductions and solvers
     DevBuf d_partials, d_result;
     std::vector<DevBuf *> scratch_vectors;

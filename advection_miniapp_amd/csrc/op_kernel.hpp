@@ -185,6 +185,20 @@ namespace dgops
       v[i] = buf[Lay<dim, n>::template off<dir>(p, i)];
   }
 
+  // row `row` of a cell's geometry table [dim + 2][nd], read along line p
+  // of direction dir
+  template <int dim, int n, int dir, typename T>
+  __device__ __forceinline__ void
+  table_line(const T *tab, int64_t cell, int row, int p, bool active,
+             T (&v)[n])
+  {
+    constexpr int nd   = Lay<dim, n>::nd;
+    const T      *base = tab + (cell * (dim + 2) + row) * nd;
+#pragma unroll
+    for (int i = 0; i < n; ++i)
+      v[i] = active ? base[Lay<dim, n>::template off<dir>(p, i)] : (T)0;
+  }
+
   // ---- geometry ------------------------------------------------------------
 
   // Per quadrature point data of the cell term: cv[d] = w * (adj(J) a)_d
@@ -298,6 +312,7 @@ namespace dgops
              const DevBasis<double> *Bd,
              const DevBasis<T>      &B,
              const double           *X,
+             int64_t                 cell,
              int                     p,
              bool                    active,
              const int (&nb)[2 * dim],
@@ -323,9 +338,24 @@ namespace dgops
         T up = uplus[f];
         if (active && nb[f] < 0)
           {
-            double an, x[3];
-            face_point_data<dim, cartesian>(P, X, Bd->xq, Bd->w, dir, s, q,
-                                            an, x);
+            double x[3] = {0, 0, 0};
+            if constexpr (cartesian)
+              {
+                double an;
+                face_point_data<dim, cartesian>(P, X, Bd->xq, Bd->w, dir, s,
+                                                q, an, x);
+              }
+            else
+              {
+                const double *rec =
+                  P.tab_bnd + ((int64_t)P.bnd_index[cell * 2 * dim + f] *
+                                 Lay<dim, n>::nl +
+                               p) *
+                                (1 + dim);
+#pragma unroll
+                for (int e = 0; e < dim; ++e)
+                  x[e] = rec[1 + e];
+              }
             up = (T)analytic_solution<dim>(P.sol_kind, P.sol_param,
                                            P.sol_time, P.vel_const, x);
           }
@@ -448,15 +478,21 @@ namespace dgops
     constexpr int LAST = dim - 1;
     constexpr int NX   = cartesian ? dim : NV * dim;
 
+    // analytic (axis-aligned) geometry evaluates the transport speed at
+    // LAST-line points and hands the other directions over through LDS;
+    // tabulated geometry reads every direction in its own line layout
+    constexpr bool AN = cartesian;
     __shared__ T      s_u[CPB * STR];
     __shared__ T      s_acc[CPB * STR];
-    __shared__ T      s_g0[CPB * STR];
-    __shared__ T      s_g1[dim == 3 ? CPB * STR : 1];
-    __shared__ T      s_sk0[skew ? CPB * STR : 1];
-    __shared__ T      s_sk1[(skew && dim == 3) ? CPB * STR : 1];
+    __shared__ T      s_g0[AN ? CPB * STR : 1];
+    __shared__ T      s_g1[(AN && dim == 3) ? CPB * STR : 1];
+    __shared__ T      s_sk0[(AN && skew) ? CPB * STR : 1];
+    __shared__ T      s_sk1[(AN && skew && dim == 3) ? CPB * STR : 1];
     __shared__ T      s_face[dim == 3 ? CPB * FSTR : 1];
-    __shared__ T      s_fan[CPB * 2 * (dim - 1) * nl];
-    __shared__ double s_X[CPB * NX];
+    __shared__ T      s_fan[AN ? CPB * 2 * (dim - 1) * nl : 1];
+    __shared__ double s_X[AN ? CPB * NX : 1];
+    const T *tab_c = static_cast<const T *>(P.tab_cell);
+    const T *tab_f = static_cast<const T *>(P.tab_face);
 
     const DevBasis<T> &B    = *Bp;
     const int          tid  = threadIdx.x;
@@ -468,15 +504,15 @@ namespace dgops
 
     T      *u_l   = s_u + lcs * STR;
     T      *acc_l = s_acc + lcs * STR;
-    T      *g0_l  = s_g0 + lcs * STR;
-    T      *g1_l  = dim == 3 ? s_g1 + lcs * STR : s_g1;
-    T      *sk0_l = skew ? s_sk0 + lcs * STR : s_sk0;
-    T      *sk1_l = (skew && dim == 3) ? s_sk1 + lcs * STR : s_sk1;
+    T      *g0_l  = AN ? s_g0 + lcs * STR : s_g0;
+    T      *g1_l  = (AN && dim == 3) ? s_g1 + lcs * STR : s_g1;
+    T      *sk0_l = (AN && skew) ? s_sk0 + lcs * STR : s_sk0;
+    T      *sk1_l = (AN && skew && dim == 3) ? s_sk1 + lcs * STR : s_sk1;
     T      *f_l   = dim == 3 ? s_face + lcs * FSTR : s_face;
-    T      *fan_l = s_fan + lcs * 2 * (dim - 1) * nl;
-    double *X     = s_X + lcs * NX;
+    T      *fan_l = AN ? s_fan + lcs * 2 * (dim - 1) * nl : s_fan;
+    double *X     = AN ? s_X + lcs * NX : s_X;
 
-    if (active)
+    if (AN && active)
       for (int e = p; e < NX; e += nl)
         X[e] = P.cell_vertices[cell * NV * dim + e];
     int nb[2 * dim];
@@ -649,6 +685,47 @@ namespace dgops
     // handed over through LDS
     T cv[3][n], jxw[n];
     T fan[2 * dim];
+    if constexpr (!AN)
+      {
+        // tabulated: LAST-direction speed and JxW on this thread's line,
+        // a . n dA on the faces it touches (lower face of this cell, upper
+        // face = lower face of the upper neighbour, boundary faces apart)
+        const int64_t tc = cell * (dim + 2) * nd;
+#pragma unroll
+        for (int i = 0; i < n; ++i)
+          {
+            const int o = L::template off<LAST>(p, i);
+            cv[LAST][i] = active ? tab_c[tc + LAST * nd + o] : (T)0;
+            jxw[i] = (active && P.has_mass) ? tab_c[tc + dim * nd + o] : (T)1;
+          }
+#pragma unroll
+        for (int d = 0; d < dim; ++d)
+          {
+            fan[2 * d] =
+              active ? -tab_f[(cell * dim + d) * nl + p] : (T)0;
+            T up = 0;
+            if (active)
+              {
+                const int nbu = nb[2 * d + 1];
+                if (nbu >= 0)
+                  up = tab_f[((int64_t)nbu * dim + d) * nl + p];
+                else
+                  up = (T)P.tab_bnd[((int64_t)P.bnd_index[cell * 2 * dim +
+                                                          2 * d + 1] *
+                                       nl +
+                                     p) *
+                                    (1 + dim)];
+              }
+            fan[2 * d + 1] = up;
+            if (active && nb[2 * d] < 0)
+              fan[2 * d] =
+                (T)P.tab_bnd[((int64_t)P.bnd_index[cell * 2 * dim + 2 * d] *
+                                nl +
+                              p) *
+                             (1 + dim)];
+          }
+      }
+    else
     {
       int q[3];
       L::template idx<LAST>(p, 0, q);
@@ -686,7 +763,9 @@ namespace dgops
 #pragma unroll
     for (int c = 0; c < NC; ++c)
       {
-        const T vf = (T)P.vel_factor[c];
+        const T vf    = (T)P.vel_factor[c];
+        const T fcell = (T)(-(1. - P.skew) * P.sign) * vf;
+        const T fs    = (T)(P.skew * P.sign) * vf;
         T       acc[n];
         // ---- LAST direction (z in 3D, y in 2D) ----
         {
@@ -704,8 +783,7 @@ namespace dgops
                     acc[i] = fma(m * jxw[i], uqs[b][i], acc[i]);
                 }
             }
-          const T fcell = (T)(-(1. - P.skew) * P.sign) * vf;
-          T       g[n];
+          T g[n];
 #pragma unroll
           for (int i = 0; i < n; ++i)
             g[i] = fcell * cv[LAST][i] * uqs[c][i];
@@ -714,43 +792,45 @@ namespace dgops
             {
               T du[n];
               mat_apply<n>(B.D, uqs[c], du);
-              const T fs = (T)(P.skew * P.sign) * vf;
 #pragma unroll
               for (int i = 0; i < n; ++i)
                 acc[i] = fma(fs * cv[LAST][i], du[i], acc[i]);
             }
           if constexpr (diff)
             diffusion_cell_term<dim, n, LAST, T>(P, Bd, B, p, uqs[c], acc);
-          face_terms<dim, n, LAST, T, cartesian, diff>(P, Bd, B, X, p, active,
+          face_terms<dim, n, LAST, T, cartesian, diff>(P, Bd, B, X, cell, p,
+                                                       active,
                                                        nb, uplus[c], dplus[c],
                                                        uqs[c], acc, vf, fan,
                                                        c == 0);
           lds_write_line<dim, n, LAST>(u_l, p, uqs[c]);
           lds_write_line<dim, n, LAST>(acc_l, p, acc);
-#pragma unroll
-          for (int i = 0; i < n; ++i)
-            g[i] = fcell * cv[0][i] * uqs[c][i];
-          lds_write_line<dim, n, LAST>(g0_l, p, g);
-          if constexpr (dim == 3)
+          if constexpr (AN)
             {
 #pragma unroll
               for (int i = 0; i < n; ++i)
-                g[i] = fcell * cv[1][i] * uqs[c][i];
-              lds_write_line<dim, n, LAST>(g1_l, p, g);
-            }
-          if constexpr (skew)
-            {
-              const T fs = (T)(P.skew * P.sign) * vf;
-#pragma unroll
-              for (int i = 0; i < n; ++i)
-                g[i] = fs * cv[0][i];
-              lds_write_line<dim, n, LAST>(sk0_l, p, g);
+                g[i] = fcell * cv[0][i] * uqs[c][i];
+              lds_write_line<dim, n, LAST>(g0_l, p, g);
               if constexpr (dim == 3)
                 {
 #pragma unroll
                   for (int i = 0; i < n; ++i)
-                    g[i] = fs * cv[1][i];
-                  lds_write_line<dim, n, LAST>(sk1_l, p, g);
+                    g[i] = fcell * cv[1][i] * uqs[c][i];
+                  lds_write_line<dim, n, LAST>(g1_l, p, g);
+                }
+              if constexpr (skew)
+                {
+#pragma unroll
+                  for (int i = 0; i < n; ++i)
+                    g[i] = fs * cv[0][i];
+                  lds_write_line<dim, n, LAST>(sk0_l, p, g);
+                  if constexpr (dim == 3)
+                    {
+#pragma unroll
+                      for (int i = 0; i < n; ++i)
+                        g[i] = fs * cv[1][i];
+                      lds_write_line<dim, n, LAST>(sk1_l, p, g);
+                    }
                 }
             }
           __syncthreads();
@@ -759,23 +839,37 @@ namespace dgops
         // ---- y direction in 3D ----
         if constexpr (dim == 3)
           {
-            T uq[n], g[n];
+            T uq[n], g[n], cvd[n];
             lds_read_line<dim, n, 1>(u_l, p, uq);
-            lds_read_line<dim, n, 1>(g1_l, p, g);
+            if constexpr (AN)
+              lds_read_line<dim, n, 1>(g1_l, p, g);
+            else
+              {
+                table_line<dim, n, 1>(tab_c, cell, 1, p, active, cvd);
+#pragma unroll
+                for (int i = 0; i < n; ++i)
+                  g[i] = fcell * cvd[i] * uq[i];
+              }
             lds_read_line<dim, n, 1>(acc_l, p, acc);
             mat_apply_t_add<n>(B.D, g, acc);
             if constexpr (skew)
               {
                 T du[n], sk[n];
                 mat_apply<n>(B.D, uq, du);
-                lds_read_line<dim, n, 1>(sk1_l, p, sk);
+                if constexpr (AN)
+                  lds_read_line<dim, n, 1>(sk1_l, p, sk);
+                else
+#pragma unroll
+                  for (int i = 0; i < n; ++i)
+                    sk[i] = fs * cvd[i];
 #pragma unroll
                 for (int i = 0; i < n; ++i)
                   acc[i] = fma(sk[i], du[i], acc[i]);
               }
             if constexpr (diff)
               diffusion_cell_term<dim, n, 1, T>(P, Bd, B, p, uq, acc);
-            face_terms<dim, n, 1, T, cartesian, diff>(P, Bd, B, X, p, active,
+            face_terms<dim, n, 1, T, cartesian, diff>(P, Bd, B, X, cell, p,
+                                                      active,
                                                       nb, uplus[c], dplus[c],
                                                       uq, acc, vf, fan,
                                                       c == 0);
@@ -785,37 +879,56 @@ namespace dgops
 
         // ---- x direction ----
         {
-          T uq[n], g[n];
+          T uq[n], g[n], cvd[n];
           lds_read_line<dim, n, 0>(u_l, p, uq);
-          lds_read_line<dim, n, 0>(g0_l, p, g);
+          if constexpr (AN)
+            lds_read_line<dim, n, 0>(g0_l, p, g);
+          else
+            {
+              table_line<dim, n, 0>(tab_c, cell, 0, p, active, cvd);
+#pragma unroll
+              for (int i = 0; i < n; ++i)
+                g[i] = fcell * cvd[i] * uq[i];
+            }
           lds_read_line<dim, n, 0>(acc_l, p, acc);
           mat_apply_t_add<n>(B.D, g, acc);
           if constexpr (skew)
             {
               T du[n], sk[n];
               mat_apply<n>(B.D, uq, du);
-              lds_read_line<dim, n, 0>(sk0_l, p, sk);
+              if constexpr (AN)
+                lds_read_line<dim, n, 0>(sk0_l, p, sk);
+              else
+#pragma unroll
+                for (int i = 0; i < n; ++i)
+                  sk[i] = fs * cvd[i];
 #pragma unroll
               for (int i = 0; i < n; ++i)
                 acc[i] = fma(sk[i], du[i], acc[i]);
             }
           if constexpr (diff)
             diffusion_cell_term<dim, n, 0, T>(P, Bd, B, p, uq, acc);
-          face_terms<dim, n, 0, T, cartesian, diff>(P, Bd, B, X, p, active,
+          face_terms<dim, n, 0, T, cartesian, diff>(P, Bd, B, X, cell, p,
+                                                    active,
                                                     nb, uplus[c], dplus[c], uq,
                                                     acc, vf, fan, c == 0);
         }
         __syncthreads();
         if constexpr (fin == fin_inverse_mass)
           {
-            // 1/JxW sits at LAST-line points; move it to x-lines via LDS
             T ij[n];
+            if constexpr (AN)
+              {
+                // 1/JxW sits at LAST-line points; move it to x-lines
 #pragma unroll
-            for (int i = 0; i < n; ++i)
-              ij[i] = (T)1 / jxw[i];
-            lds_write_line<dim, n, LAST>(g0_l, p, ij);
-            __syncthreads();
-            lds_read_line<dim, n, 0>(g0_l, p, ij);
+                for (int i = 0; i < n; ++i)
+                  ij[i] = (T)1 / jxw[i];
+                lds_write_line<dim, n, LAST>(g0_l, p, ij);
+                __syncthreads();
+                lds_read_line<dim, n, 0>(g0_l, p, ij);
+              }
+            else
+              table_line<dim, n, 0>(tab_c, cell, dim + 1, p, active, ij);
 #pragma unroll
             for (int i = 0; i < n; ++i)
               acc[i] *= ij[i];

@@ -189,6 +189,16 @@ class AdvectionOperation:
         L.check(self._lib.dg_irk_precondition_block_jacobi(
             self._h, self._v(dst, "dst", s), self._v(src, "src", s)))
 
+    def irk_solve(self, stage, rhs, control):
+        """(F)GMRES on the stage system from the initial guess in `stage`
+        (SolverFGMRES::solve, advection_solver_irk_real.cc:2729-2757);
+        `control` (dg_solver_control) returns iterations and residuals."""
+        s = self.n_stages
+        L.check(self._lib.dg_irk_solve(self._h, self._v(stage, "stage", s),
+                                       self._v(rhs, "rhs", s),
+                                       ctypes.byref(control)))
+        return control
+
     # ---- index data -----------------------------------------------------------
     def cell_coordinates(self):
         a = np.zeros((self.n_cells, 3), dtype=np.int32)

@@ -1,14 +1,23 @@
-"""Headline benchmark: DoF/s and fraction of the HBM roofline of the 3D,
-degree-4, FP64 DG operator apply on one MI355X.
+"""Headline benchmark on one MI355X.
 
-Workload (the metric's configuration): 64^3 cells on [0,1]^3 with interior
-vertices perturbed by up to 10% of h (seed 42), degree 4 on Gauss-Lobatto
-nodes (32,768,000 DoF), periodic, divergence-free velocity
+Metric: DoF/s and fraction of the HBM roofline of the 3D, degree-4, FP64 DG
+operator apply; and the time of the mixed-precision FDM-preconditioned IRK
+solve to a 1e-10 relative residual against the all-FP64 solve.
+
+Apply workload (the metric's configuration): 64^3 cells on [0,1]^3 with
+interior vertices perturbed by up to 10% of h (seed 42), degree 4 on
+Gauss-Lobatto nodes (32,768,000 DoF), periodic, divergence-free velocity
 (sin2πy cos2πz, sin2πz cos2πx, sin2πx cos2πy), upwind flux, Gaussian pulse
 initial data of width 0.1. One step = one operator apply
-dst = M^-1(-L src) over the whole mesh with src/dst resident in HBM.
+dst = M^-1(-L src) over the whole mesh, src/dst resident in HBM.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+IRK workload: 32^3 cells on [0,1]^3, degree 4, the same velocity field,
+Radau IIA with 3 stages (12,288,000 unknowns), dt = 5 h_min / 2 (Courant
+number 5, advection_solver_irk_real.cc:2352), flexible GMRES(30) in FP64 to
+a 1e-10 relative residual from a zero initial guess, cellwise FDM
+preconditioner in FP64 against the FP32 variants.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-irk]
 
 Prints one JSON line.
 """
@@ -18,28 +27,40 @@ import os
 import sys
 import time
 
-import numpy as np
-
+ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
 FP64_PEAK_TFLOPS = 78.6
+# one 1D sweep over n^d values costs 2 n^(d+1) flop; the apply runs 9 cell
+# sweeps and 6 face interpolations/integrations of 2 (d-1)-dimensional
+# sweeps each, element-centric
+APPLY_FLOP_PER_DOF = (9 * 2 * 5 ** 4 + 6 * 3 * 2 * 5 ** 3) / 5 ** 3
 
 
-def workload(refine=6, degree=4, dim=3):
+def apply_workload(refine=6, degree=4, dim=3):
     return dict(dim=dim, degree=degree, refine=refine, perturbation=0.1,
                 seed=42, velocity_kind=1, flux_alpha=1.0, factor_skew=0.0,
                 solution_kind=3, solution_param=[0.5, 0.5, 0.5, 0.1],
                 periodic=[1, 1, 1], lo=[0, 0, 0], hi=[1, 1, 1])
 
 
+def irk_workload(refine=5):
+    return dict(dim=3, degree=4, refine=refine, perturbation=0.0,
+                velocity_kind=1, flux_alpha=1.0, factor_skew=0.5,
+                irk_stages=3, solution_kind=3,
+                solution_param=[0.5, 0.5, 0.5, 0.1],
+                periodic=[1, 1, 1], lo=[0, 0, 0], hi=[1, 1, 1])
+
+
 def cpu_baseline(threads=None, target_seconds=15.0):
     """The oracle's apply (restated reference algorithm, OpenMP over cells)
-    on a bounded sample: 16^3 cells of the same workload."""
-    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    on a bounded sample: 16^3 cells of the apply workload."""
+    sys.path.insert(0, ROOT)
     import oracle
     from tests.helpers import Cfg
     cores = threads or len(os.sched_getaffinity(0))
+    cores = min(cores, 16)
     oracle.load().or_set_threads(cores)
-    o = Cfg(**workload(refine=4)).oracle_handle()
+    o = Cfg(**apply_workload(refine=4)).oracle_handle()
     u = o.project(0.0)
     o.apply(u, 0.0)  # warm up
     reps, t0 = 0, time.perf_counter()
@@ -47,13 +68,105 @@ def cpu_baseline(threads=None, target_seconds=15.0):
         o.apply(u, 0.0)
         reps += 1
         el = time.perf_counter() - t0
-        if el > target_seconds or reps >= 50:
+        if el > target_seconds or reps >= 200:
             break
     return dict(value=o.n_dofs * reps / el, unit="DoF/s", cores=cores,
                 kind="port",
                 sample=f"{reps} applies on 16^3 cells (degree 4, "
-                       f"{o.n_dofs} DoF) of the same workload, restated "
-                       "reference algorithm (oracle/oracle.cpp), not deal.II")
+                       f"{o.n_dofs} DoF) of the same workload in {el:.1f} s, "
+                       "restated reference algorithm (oracle/oracle.cpp, "
+                       "OpenMP), not deal.II")
+
+
+def measured_traffic():
+    """HBM bytes per apply launch from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated),
+    or None when no summary exists."""
+    path = os.path.join(ROOT, "profiles", "apply_3d_k4_64cube_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def time_apply(args):
+    import torch
+    from advection_miniapp_amd import AdvectionOperation, make_config
+    op = AdvectionOperation(make_config(**apply_workload(refine=args.refine)))
+    src = op.initialize_dof_vector()
+    op.project_initial(src, 0.0)
+    dst = op.initialize_dof_vector()
+    for _ in range(max(1, args.warmup)):
+        op.apply(src, dst, 0.0)
+    torch.cuda.synchronize()
+    # the library launches on torch's current stream (op._stream), so
+    # events recorded there bracket exactly the K operator kernels
+    stream = torch.cuda.current_stream()
+    assert stream.cuda_stream == op._stream
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(args.steps):
+        op.apply(src, dst, 0.0)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / args.steps
+    return op, ms
+
+
+def time_irk(args):
+    import torch
+    from advection_miniapp_amd import AdvectionOperation, make_config
+    from advection_miniapp_amd import _lib as L
+    from advection_miniapp_amd.operators import solver_control
+    cfg = irk_workload(refine=args.irk_refine)
+    op = AdvectionOperation(make_config(**cfg))
+    s = 3
+    u = op.initialize_dof_vector()
+    op.project_initial(u, 0.0)
+    h = 1.0 / 2 ** args.irk_refine
+    dt = 5.0 * h / 2.0
+    op.set_time(0.0, dt)
+    rhs = op.initialize_dof_vector(blocks=s)
+    op.compute_rhs(rhs, u)
+    rn = rhs.norm().item()
+    variants = {
+        "fp64": dict(prec_precision=L.DG_FP64),
+        "fp32_fdm": dict(prec_precision=L.DG_FP32),
+        "fp32_inner4": dict(prec_precision=L.DG_FP32, inner_iterations=4),
+    }
+    out, sol = {}, {}
+    for name, kw in variants.items():
+        times, z = [], None
+        for rep in range(1 + args.irk_reps):
+            z = op.initialize_dof_vector(blocks=s)
+            ctl = solver_control(max_iterations=400, rel_tolerance=1e-10,
+                                 restart=30, **kw)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            op.irk_solve(z, rhs, ctl)
+            torch.cuda.synchronize()
+            if rep > 0:      # the first solve builds tables and buffers
+                times.append(time.perf_counter() - t0)
+        res = op.initialize_dof_vector(blocks=s)
+        op.vmult_irk(res, z)
+        res -= rhs
+        times.sort()
+        out[name] = dict(ms=1e3 * times[len(times) // 2],
+                         iterations=ctl.iterations,
+                         rel_residual=res.norm().item() / rn)
+        sol[name] = z
+    ref = sol["fp64"]
+    for name in variants:
+        out[name]["rel_diff_vs_fp64"] = \
+            ((sol[name] - ref).norm() / ref.norm()).item()
+    best = min((n for n in variants if n != "fp64"),
+               key=lambda n: out[n]["ms"])
+    return dict(workload=f"3D IRK Radau IIA(3), degree 4, "
+                         f"{2 ** args.irk_refine}^3 cells, Courant 5, "
+                         "FGMRES(30) FP64 to 1e-10 from zero",
+                unknowns=op.n_dofs * s, mixed=best,
+                speedup=out["fp64"]["ms"] / out[best]["ms"], **out)
 
 
 def main():
@@ -62,7 +175,10 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--refine", type=int, default=6)
+    ap.add_argument("--irk-refine", type=int, default=5)
+    ap.add_argument("--irk-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-irk", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box of this build has one "
@@ -70,34 +186,15 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    import torch
-    from advection_miniapp_amd import AdvectionOperation, make_config
-
-    cfg = make_config(**workload(refine=args.refine))
-    op = AdvectionOperation(cfg)
-    src = op.initialize_dof_vector()
-    op.project_initial(src, 0.0)
-    dst = op.initialize_dof_vector()
-    for _ in range(args.warmup):
-        op.apply(src, dst, 0.0)
-    torch.cuda.synchronize()
-    start, stop = torch.cuda.Event(enable_timing=True), \
-        torch.cuda.Event(enable_timing=True)
-    t_wall = time.perf_counter()
-    start.record()
-    for _ in range(args.steps):
-        op.apply(src, dst, 0.0)
-    stop.record()
-    torch.cuda.synchronize()
-    t_wall = time.perf_counter() - t_wall
-    ms = start.elapsed_time(stop) / args.steps
+    op, ms = time_apply(args)
     n = op.n_dofs
-    dofs_per_s = n / (ms * 1e-3)
     bytes_per_apply = 16.0 * n
     achieved = bytes_per_apply / (ms * 1e-3) / 1e9
+    traffic = measured_traffic()
     out = {
-        "metric": "DoF/s & %HBM peak, 3D p=4 DG apply",
-        "value": dofs_per_s,
+        "metric": "DoF/s & %HBM peak, 3D p=4 DG apply; mixed vs FP64 IRK "
+                  "time at 1e-10 res",
+        "value": n / (ms * 1e-3),
         "unit": "DoF/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -108,13 +205,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "3D advection, degree 4, 64^3 perturbed "
-                               "cells, cellular velocity, FP64 apply",
+        "config": {"workload": "3D advection apply, degree 4, 64^3 cells "
+                               "perturbed 10% (seed 42), cellular velocity, "
+                               "FP64",
                    "n_dofs": n, "refine": args.refine},
         "roofline": {"bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+        "fp64_tflops": APPLY_FLOP_PER_DOF * n / (ms * 1e-3) / 1e12,
     }
+    del op
+    if not args.no_irk:
+        out["irk"] = time_irk(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

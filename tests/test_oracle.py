@@ -324,3 +324,67 @@ def test_burgers_conserves_momentum():
         comp = np.ascontiguousarray(du[:, c, :]).ravel()
         m = o.mass(comp)
         assert abs(m.sum()) < 1e-11 * np.abs(m).sum()
+
+
+# ---- implicit and IRK operators: compositions of pinned pieces ------------
+
+IMPLICIT = dict(dim=2, degree=3, refine=2, perturbation=0.15,
+                velocity_kind=2, time_factor=1, final_time=4.0,
+                factor_skew=0.5)
+
+
+def test_implicit_vmult_is_mass_over_dt_plus_operator():
+    """(M/dt + L(t+dt)) u from its two independently pinned parts, the mass
+    matrix and the explicit loop (which returns -L u)."""
+    o = Cfg(**IMPLICIT).oracle_handle()
+    u = np.random.default_rng(11).standard_normal(o.n_dofs)
+    t, dt = 0.3, 0.07
+    ref = o.mass(u) / dt - o.loop(u, t + dt)
+    assert rel_diff(o.vmult(u, t, dt), ref) < 1e-13
+
+
+@pytest.mark.parametrize("s", [2, 3])
+def test_irk_vmult_is_stage_coupled_mass_plus_operator(s):
+    """Block c of the IRK operator: sum_b (A^-1)_cb M Z_b / dt + L(t+c_c dt)
+    Z_c, assembled from the mass matrix, the explicit loop and the Radau IIA
+    coefficients."""
+    o = Cfg(irk_stages=s, **IMPLICIT).oracle_handle()
+    N = o.n_dofs
+    Z = np.random.default_rng(12).standard_normal(N * s).reshape(s, N)
+    t, dt = 0.2, 0.05
+    A, b, c = oracle.irk_coefficients(s)
+    Ainv = np.linalg.inv(A)
+    MZ = np.array([o.mass(z) for z in Z])
+    ref = np.array([Ainv[i] @ MZ / dt - o.loop(Z[i], t + c[i] * dt)
+                    for i in range(s)])
+    assert rel_diff(o.irk_vmult(Z.ravel(), t, dt), ref.ravel()) < 1e-13
+    rhs = np.array([o.loop(Z[0], t + c[i] * dt) / dt for i in range(s)])
+    assert rel_diff(o.irk_compute_rhs(Z[0], t, dt), rhs.ravel()) < 1e-13
+
+
+def test_irk_step_converges_with_order_2s_minus_1():
+    """Radau IIA(3) steps of the oracle's compute_rhs / vmult /
+    update_solution against the exact solution exp(-T M^-1 L) u0 of the
+    semi-discrete system: the error falls like dt^5."""
+    from scipy.linalg import expm
+    s = 3
+    c = Cfg(dim=2, degree=2, refine=1, velocity=[1.0, 0.5], factor_skew=0.5,
+            irk_stages=s, solution_kind=2, solution_param=[2 * np.pi, 0])
+    o = c.oracle_handle()
+    N = o.n_dofs
+    u0 = o.project(0.0)
+    Mh = np.column_stack([o.apply(e, 0.0) for e in np.eye(N)])
+    T = 0.4
+    exact = expm(T * Mh) @ u0
+    errs = []
+    for nsteps in (4, 8, 16):
+        dt = T / nsteps
+        Big = np.column_stack([o.irk_vmult(e, 0.0, dt)
+                               for e in np.eye(N * s)])
+        u = u0.copy()
+        for _ in range(nsteps):
+            Z = np.linalg.solve(Big, o.irk_compute_rhs(u, 0.0, dt))
+            o.irk_update_solution(u, Z, dt)
+        errs.append(np.linalg.norm(u - exact))
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all((rates > 4.5) & (rates < 5.6)), (errs, rates)
