@@ -588,6 +588,28 @@ dg_lsrk_n_stages(int scheme)
 }
 
 int
+dg_lsrk_coefficients(int scheme, double *bi, double *ai)
+{
+  std::vector<double> b, a;
+  try
+    {
+      lsrk_coefficients(scheme, b, a);
+    }
+  catch (...)
+    {
+      return -1;
+    }
+  if (!bi || !ai)
+    return -1;
+  for (size_t i = 0; i < b.size(); ++i)
+    {
+      bi[i] = b[i];
+      ai[i] = i < a.size() ? a[i] : 0.;
+    }
+  return (int)b.size();
+}
+
+int
 dg_lsrk_time_step(dg_operator *h, int scheme, double t, double dt,
                   void *solution, void *vec_ri, void *vec_ki)
 {

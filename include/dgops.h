@@ -176,6 +176,11 @@ int dg_lsrk_time_step(dg_operator *op,
                       void        *vec_ri,
                       void        *vec_ki);
 int dg_lsrk_n_stages(int scheme);
+/* the scheme's b_i and a_i (n_stages values each, a_i of the last stage
+ * 0), for integrators written on top of dg_perform_stage; returns the
+ * number of stages or -1          LowStorageRungeKuttaIntegrator
+ *                                  advection_solver.cc:828-897          */
+int dg_lsrk_coefficients(int scheme, double *bi, double *ai);
 
 /* L2 projection of the analytic solution at time t with QGauss(k+1)
  *                      AdvectionOperation::project_initial advection_solver.cc:760-784 */
