@@ -23,6 +23,10 @@ namespace dgops
 
   // geometry tables of the operator kernels in precision p (see OpParams)
   void aux_geometry_tables(Operator &op, int p);
+  // analytic Dirichlet data (P.sol_kind at P.sol_time) on every boundary
+  // face point, for the tabulated-geometry kernels (no-op without boundary
+  // faces)
+  void aux_boundary_values(Operator &op, const OpParams &P);
 
   // vector operations, precision p (DG_FP64 / DG_FP32)
   void vec_axpby(Operator &op, int p, int64_t n, double a, const void *x,

@@ -805,6 +805,20 @@ namespace dgops
         }
     }
 
+    template <int dim>
+    __global__ void
+    boundary_value_kernel(const OpParams P, int64_t count,
+                          const double *__restrict__ tab, double *out)
+    {
+      const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+      if (i >= count)
+        return;
+      const double *r    = tab + i * (1 + dim);
+      const double  x[3] = {r[1], r[2], dim == 3 ? r[3] : 0.};
+      out[i] = analytic_solution<dim>(P.sol_kind, P.sol_param, P.sol_time,
+                                      P.vel_const, x);
+    }
+
     template <int dim, int n>
     void
     launch_geometry_tables(Operator &op, const OpParams &P, const int *bcell,
@@ -926,6 +940,8 @@ namespace dgops
                                std::max<int64_t>(1, nc));
         op.d_tab_bnd.alloc(sizeof(double) * (1 + dim) * nl *
                            std::max<int64_t>(1, nb));
+        op.d_bnd_value.alloc(sizeof(double) * nl * std::max<int64_t>(1, nb));
+        op.n_bnd = nb;
         const OpParams P = op.base_params(0, 1);
 #define DG_GT(D, N)                                                         \
   else if (dim == D && n == N)                                              \
@@ -953,6 +969,25 @@ namespace dgops
                          op.d_tab_face[1].ptr);
         op.tables_ready[1] = true;
       }
+  }
+
+  void
+  aux_boundary_values(Operator &op, const OpParams &P)
+  {
+    if (op.n_bnd == 0)
+      return;
+    const int     nl    = op.cfg.dim == 3 ? op.n * op.n : op.n;
+    const int64_t count = op.n_bnd * nl;
+    const unsigned nb   = (unsigned)((count + aux_threads - 1) / aux_threads);
+    if (op.cfg.dim == 3)
+      hipLaunchKernelGGL(boundary_value_kernel<3>, dim3(nb), dim3(aux_threads),
+                         0, op.stream, P, count, op.d_tab_bnd.as<double>(),
+                         op.d_bnd_value.as<double>());
+    else
+      hipLaunchKernelGGL(boundary_value_kernel<2>, dim3(nb), dim3(aux_threads),
+                         0, op.stream, P, count, op.d_tab_bnd.as<double>(),
+                         op.d_bnd_value.as<double>());
+    hip_check(hipGetLastError(), "boundary values");
   }
 
   void

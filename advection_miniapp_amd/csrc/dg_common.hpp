@@ -56,6 +56,7 @@ namespace dgops
     const void   *tab_face;
     const double *tab_bnd;
     const int    *bnd_index; // [cell][2 dim] boundary face id, -1 inside
+    const double *bnd_value; // [boundary face][nl] Dirichlet data at sol_time
   };
 
   // LSRK stage update (advection_solver.cc:728-751)

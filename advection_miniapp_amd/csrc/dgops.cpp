@@ -230,6 +230,8 @@ namespace dgops
         Pk.tab_face  = m.d_tab_face[fp32 ? 1 : 0].ptr;
         Pk.tab_bnd   = m.d_tab_bnd.as<double>();
         Pk.bnd_index = m.d_bnd_index.as<int>();
+        Pk.bnd_value = m.d_bnd_value.as<double>();
+        aux_boundary_values(m, Pk);
       }
     const uint32_t key = op_key(op.cfg.dim, op.n, nc, fp32 ? 1 : 0,
                                 analytic ? 1 : 0, skew ? 1 : 0, diff, fin,

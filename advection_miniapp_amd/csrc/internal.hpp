@@ -113,6 +113,9 @@ namespace dgops
     // is applied in the kernel)
     DevBuf d_tab_cell[2], d_tab_face[2], d_tab_bnd, d_bnd_index;
     bool   tables_ready[2] = {false, false};
+    // Dirichlet data at the boundary face points, refreshed per launch
+    DevBuf  d_bnd_value;
+    int64_t n_bnd = 0; // boundary faces
 
     // scratch for reductions and solvers
     DevBuf d_partials, d_result;

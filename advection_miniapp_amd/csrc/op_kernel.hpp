@@ -338,26 +338,19 @@ namespace dgops
         T up = uplus[f];
         if (active && nb[f] < 0)
           {
-            double x[3] = {0, 0, 0};
             if constexpr (cartesian)
               {
-                double an;
+                double an, x[3] = {0, 0, 0};
                 face_point_data<dim, cartesian>(P, X, Bd->xq, Bd->w, dir, s,
                                                 q, an, x);
+                up = (T)analytic_solution<dim>(P.sol_kind, P.sol_param,
+                                               P.sol_time, P.vel_const, x);
               }
             else
-              {
-                const double *rec =
-                  P.tab_bnd + ((int64_t)P.bnd_index[cell * 2 * dim + f] *
-                                 Lay<dim, n>::nl +
-                               p) *
-                                (1 + dim);
-#pragma unroll
-                for (int e = 0; e < dim; ++e)
-                  x[e] = rec[1 + e];
-              }
-            up = (T)analytic_solution<dim>(P.sol_kind, P.sol_param,
-                                           P.sol_time, P.vel_const, x);
+              // evaluated before the launch (boundary_value_kernel)
+              up = (T)P.bnd_value[(int64_t)P.bnd_index[cell * 2 * dim + f] *
+                                    Lay<dim, n>::nl +
+                                  p];
           }
         const T an = fan[f] * vf;
         const T flux =

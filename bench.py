@@ -179,6 +179,9 @@ def main():
     ap.add_argument("--irk-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-irk", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run plain copies of one vector (for the "
+                         "HBM counter calibration of rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box of this build has one "
@@ -188,6 +191,16 @@ def main():
 
     op, ms = time_apply(args)
     n = op.n_dofs
+    if args.copy_calibration:
+        # a plain device copy of the same size as one vector (read n*8 B,
+        # write n*8 B): the counter reading of this kernel calibrates the
+        # FETCH_SIZE / WRITE_SIZE readings of the operator kernel
+        import torch
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(args.steps):
+            b.copy_(a)
+        torch.cuda.synchronize()
     bytes_per_apply = 16.0 * n
     achieved = bytes_per_apply / (ms * 1e-3) / 1e9
     traffic = measured_traffic()
