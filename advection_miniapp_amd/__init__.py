@@ -9,10 +9,12 @@ from ._lib import DgError
 from .operators import (AdvectionOperation,
                         DiagonallyImplicitRungeKuttaIntegrator,
                         ImplicitRungeKuttaIntegrator,
-                        LowStorageRungeKuttaIntegrator, irk_coefficients,
-                        make_config, solver_control)
+                        LowStorageRungeKuttaIntegrator, explicit_time_step,
+                        implicit_time_step, irk_coefficients, make_config,
+                        solver_control)
 
 __all__ = ["AdvectionOperation", "LowStorageRungeKuttaIntegrator",
            "ImplicitRungeKuttaIntegrator",
            "DiagonallyImplicitRungeKuttaIntegrator", "DgError",
-           "make_config", "irk_coefficients", "solver_control", "_lib"]
+           "make_config", "irk_coefficients", "solver_control",
+           "explicit_time_step", "implicit_time_step", "_lib"]

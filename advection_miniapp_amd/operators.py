@@ -286,3 +286,19 @@ def irk_coefficients(stages):
     c = (ctypes.c_double * stages)()
     L.check(L.load().dg_irk_coefficients(stages, A, b, c))
     return (np.array(A).reshape(stages, stages), np.array(b), np.array(c))
+
+
+def explicit_time_step(min_vertex_distance, speed_norm, degree, courant,
+                       final_time):
+    """dt = C h_min / |a| / max(k, 1)^1.5, shortened so that final_time is a
+    whole number of steps (advection_solver.cc:1163-1167)."""
+    dt = courant * min_vertex_distance / speed_norm / \
+        max(1, degree) ** 1.5
+    return final_time / np.ceil(final_time / dt)
+
+
+def implicit_time_step(min_vertex_distance, courant, final_time):
+    """dt = C h_min / 2, shortened so that final_time is a whole number of
+    steps (advection_solver_irk_real.cc:2352-2354)."""
+    dt = courant * min_vertex_distance / 2.0
+    return final_time / np.ceil(final_time / dt)

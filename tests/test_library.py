@@ -45,3 +45,14 @@ def test_config_default_and_validation_without_gpu():
 def test_null_handle_is_an_error():
     lib = _lib.load()
     assert lib.dg_apply(None, None, None, 0.0) == _lib.DG_ERR_ARGUMENT
+
+
+def test_time_step_rules():
+    """Configuration 1: h_min = 2/32, |a| = |(1, 0.15)|, k = 3, Courant 0.4,
+    T = 2 gives 421 steps; the implicit rule C h_min / 2."""
+    import numpy as np
+    from advection_miniapp_amd import explicit_time_step, implicit_time_step
+    dt = explicit_time_step(2 / 32, np.hypot(1.0, 0.15), 3, 0.4, 2.0)
+    assert round(2.0 / dt) == 421 and abs(421 * dt - 2.0) < 1e-13
+    dt = implicit_time_step(1 / 32, 5.0, 1.0)
+    assert round(1.0 / dt) == 13 and dt <= 5.0 / 64
