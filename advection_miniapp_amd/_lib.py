@@ -124,6 +124,9 @@ SIGNATURES = {
     "dg_export_cell_coordinates": (_i, [_vp, ctypes.POINTER(ctypes.c_int32)]),
     "dg_export_neighbors": (_i, [_vp, ctypes.POINTER(ctypes.c_int32)]),
     "dg_export_cell_vertices": (_i, [_vp, _dp]),
+    "dg_host_index_map": (_i, [ctypes.POINTER(DgConfig),
+                               ctypes.POINTER(ctypes.c_int64), _dp,
+                               ctypes.POINTER(ctypes.c_int64)]),
 }
 
 _lib = None

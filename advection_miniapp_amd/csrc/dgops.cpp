@@ -254,6 +254,24 @@ namespace dgops
 
   namespace
   {
+    MeshDesc
+    mesh_desc(const dg_config &cfg)
+    {
+      MeshDesc md = {};
+      md.dim      = cfg.dim;
+      for (int d = 0; d < 3; ++d)
+        {
+          md.n_coarse[d] = d < cfg.dim ? cfg.n_coarse[d] : 1;
+          md.lo[d]       = cfg.lo[d];
+          md.hi[d]       = cfg.hi[d];
+          md.periodic[d] = cfg.periodic[d];
+        }
+      md.refine       = cfg.refine;
+      md.perturbation = cfg.perturbation;
+      md.seed         = cfg.seed;
+      return md;
+    }
+
     void
     validate(const dg_config &c)
     {
@@ -342,19 +360,7 @@ dg_create(const dg_config *cfg, void *stream, dg_operator **out)
     op->cfg    = *cfg;
     op->stream = (hipStream_t)stream;
     hip_check(hipGetDevice(&op->device), "hipGetDevice");
-    MeshDesc md = {};
-    md.dim      = cfg->dim;
-    for (int d = 0; d < 3; ++d)
-      {
-        md.n_coarse[d] = d < cfg->dim ? cfg->n_coarse[d] : 1;
-        md.lo[d]       = cfg->lo[d];
-        md.hi[d]       = cfg->hi[d];
-        md.periodic[d] = cfg->periodic[d];
-      }
-    md.refine       = cfg->refine;
-    md.perturbation = cfg->perturbation;
-    md.seed         = cfg->seed;
-    op->mesh        = make_mesh(md);
+    op->mesh        = make_mesh(mesh_desc(*cfg));
     op->n           = cfg->degree + 1;
     op->nd          = cfg->dim == 3 ? op->n * op->n * op->n : op->n * op->n;
     op->n_dofs      = op->mesh.n_cells * op->nd;
@@ -783,6 +789,35 @@ dg_export_cell_vertices(const dg_operator *h, double *v)
     require(v != nullptr, "null output");
     std::memcpy(v, op->mesh.cell_vertices.data(),
                 op->mesh.cell_vertices.size() * sizeof(double));
+  });
+}
+
+int
+dg_host_index_map(const dg_config *cfg, int64_t *n_cells,
+                  double *cell_centres, int64_t *dof_offsets)
+{
+  return guard([&] {
+    require(cfg && n_cells, "null argument");
+    validate(*cfg);
+    const StructuredMesh m  = make_mesh(mesh_desc(*cfg));
+    const int            n  = cfg->degree + 1;
+    const int64_t        nd = (int64_t)(cfg->dim == 3 ? n * n * n : n * n) *
+                       cfg->n_components;
+    *n_cells = m.n_cells;
+    const int nv = 1 << cfg->dim;
+    for (int64_t c = 0; c < m.n_cells; ++c)
+      {
+        if (cell_centres)
+          for (int d = 0; d < cfg->dim; ++d)
+            {
+              double s = 0;
+              for (int v = 0; v < nv; ++v)
+                s += m.cell_vertices[(c * nv + v) * cfg->dim + d];
+              cell_centres[c * cfg->dim + d] = s / nv;
+            }
+        if (dof_offsets)
+          dof_offsets[c] = c * nd;
+      }
   });
 }
 

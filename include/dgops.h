@@ -291,6 +291,14 @@ int dg_export_cell_coordinates(const dg_operator *op, int32_t *ijk);
 int dg_export_neighbors(const dg_operator *op, int32_t *neighbors);
 /* corners [n_cells][2^dim][dim]                                          */
 int dg_export_cell_vertices(const dg_operator *op, double *vertices);
+/* Host-only index map (no GPU needed): the number of cells, the centre of
+ * every cell (mean of its corners, [cell][dim]) and the offset of its
+ * first dof in the vectors. Null arrays are skipped, so a first call with
+ * both null returns n_cells. The numbering is the one of the reference's
+ * DoFHandler on the refine_global mesh (advection_solver.cc:1002-1174),
+ * printed by tests/cpp/index_map for comparison with a deal.II build.  */
+int dg_host_index_map(const dg_config *cfg, int64_t *n_cells,
+                      double *cell_centres, int64_t *dof_offsets);
 
 #ifdef __cplusplus
 }

@@ -56,3 +56,24 @@ def test_time_step_rules():
     assert round(2.0 / dt) == 421 and abs(421 * dt - 2.0) < 1e-13
     dt = implicit_time_step(1 / 32, 5.0, 1.0)
     assert round(1.0 / dt) == 13 and dt <= 5.0 / 64
+
+
+def test_host_index_map_tool_matches_oracle():
+    """tests/cpp/index_map (host only, no GPU): cell centres in vector order
+    and dof offsets equal the oracle's refine_global mesh."""
+    import os
+    import subprocess
+    import numpy as np
+    from helpers import Cfg
+    tool = os.path.join(os.path.dirname(__file__), "cpp", "index_map")
+    assert os.path.exists(tool), "run __graft_entry__.build() first"
+    out = subprocess.run([tool, "3", "4", "2", "0.1", "42"], check=True,
+                         capture_output=True, text=True).stdout
+    rows = np.array([[float(x) for x in line.split()]
+                     for line in out.splitlines() if not line.startswith("#")])
+    o = Cfg(dim=3, degree=4, refine=2, perturbation=0.1).oracle_handle()
+    v = o.cell_vertices().reshape(o.n_cells, 8, 3)
+    assert rows.shape == (o.n_cells, 5)
+    assert np.array_equal(rows[:, 0], np.arange(o.n_cells))
+    assert np.max(np.abs(rows[:, 1:4] - v.mean(axis=1))) < 1e-15
+    assert np.array_equal(rows[:, 4], np.arange(o.n_cells) * 125)
