@@ -48,10 +48,11 @@ namespace dgops
     double        viscosity;
     double        penalty;
     // geometry tables of the tabulated-geometry kernels (scalar type of
-    // the kernel): per cell [dim + 2][nd] = w adj(J) a (dim rows), JxW,
-    // 1/JxW at the Gauss points; per cell [dim][nl] = w (adj(J) a)_d on
-    // the face at xi_d = 0 (normal along +xi_d); per boundary face
-    // [nl][1 + dim] (double) = outward a . n dA and the physical point
+    // the kernel): per cell [dim][nd] = w adj(J) a at the Gauss points
+    // (JxW itself is recomputed from the corners along the lines); per
+    // cell [dim][nl] = w (adj(J) a)_d on the face at xi_d = 0 (normal
+    // along +xi_d); per boundary face [nl][1 + dim] (double) = outward
+    // a . n dA and the physical point
     const void   *tab_cell;
     const void   *tab_face;
     const double *tab_bnd;

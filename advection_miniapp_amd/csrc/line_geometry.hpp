@@ -346,4 +346,46 @@ namespace dgops
         out[i] = (T)((s ? 1. : -1.) * wj * Bd->w[i] * an);
       }
   }
+  // JxW = w_q det(J) at the n points of line p of direction dir (the point
+  // layout of Lay<dim, n>::idx). On a multilinear cell the Jacobian column
+  // along the line is constant and the other columns are linear in the
+  // line coordinate, so J is evaluated at the two line ends only.
+  template <int dim, int n, int dir, typename T, typename Layout>
+  __device__ __forceinline__ void
+  line_jxw(const double *X, const double *xq, const double *wq, int p,
+           T (&jxw)[n])
+  {
+    int q[3];
+    Layout::template idx<dir>(p, 0, q);
+    double xi[3] = {xq[q[0]], xq[q[1]], dim == 3 ? xq[q[2]] : 0.};
+    double wl    = 1.;
+#pragma unroll
+    for (int e = 0; e < dim; ++e)
+      if (e != dir)
+        wl *= wq[q[e]];
+    double x[3], J0[3][3], J1[3][3];
+    xi[dir] = 0.;
+    multilinear_map<dim>(X, xi, x, J0);
+    xi[dir] = 1.;
+    multilinear_map<dim>(X, xi, x, J1);
+#pragma unroll
+    for (int i = 0; i < n; ++i)
+      {
+        const double t = xq[i];
+        double       J[3][3];
+#pragma unroll
+        for (int a = 0; a < dim; ++a)
+#pragma unroll
+          for (int b = 0; b < dim; ++b)
+            J[a][b] = fma(t, J1[a][b] - J0[a][b], J0[a][b]);
+        double det;
+        if constexpr (dim == 3)
+          det = J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) +
+                J[1][0] * (J[2][1] * J[0][2] - J[0][1] * J[2][2]) +
+                J[2][0] * (J[0][1] * J[1][2] - J[1][1] * J[0][2]);
+        else
+          det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        jxw[i] = (T)(wl * wq[i] * det);
+      }
+  }
 } // namespace dgops

@@ -185,15 +185,15 @@ namespace dgops
       v[i] = buf[Lay<dim, n>::template off<dir>(p, i)];
   }
 
-  // row `row` of a cell's geometry table [dim + 2][nd], read along line p
-  // of direction dir
+  // row `row` of a cell's speed table [dim][nd], read along line p of
+  // direction dir
   template <int dim, int n, int dir, typename T>
   __device__ __forceinline__ void
   table_line(const T *tab, int64_t cell, int row, int p, bool active,
              T (&v)[n])
   {
     constexpr int nd   = Lay<dim, n>::nd;
-    const T      *base = tab + (cell * (dim + 2) + row) * nd;
+    const T      *base = tab + (cell * dim + row) * nd;
 #pragma unroll
     for (int i = 0; i < n; ++i)
       v[i] = active ? base[Lay<dim, n>::template off<dir>(p, i)] : (T)0;
@@ -483,7 +483,7 @@ namespace dgops
     __shared__ T      s_sk1[(AN && skew && dim == 3) ? CPB * STR : 1];
     __shared__ T      s_face[dim == 3 ? CPB * FSTR : 1];
     __shared__ T      s_fan[AN ? CPB * 2 * (dim - 1) * nl : 1];
-    __shared__ double s_X[AN ? CPB * NX : 1];
+    __shared__ double s_X[CPB * NX];
     const T *tab_c = static_cast<const T *>(P.tab_cell);
     const T *tab_f = static_cast<const T *>(P.tab_face);
 
@@ -503,9 +503,9 @@ namespace dgops
     T      *sk1_l = (AN && skew && dim == 3) ? s_sk1 + lcs * STR : s_sk1;
     T      *f_l   = dim == 3 ? s_face + lcs * FSTR : s_face;
     T      *fan_l = AN ? s_fan + lcs * 2 * (dim - 1) * nl : s_fan;
-    double *X     = AN ? s_X + lcs * NX : s_X;
+    double *X     = s_X + lcs * NX;
 
-    if (AN && active)
+    if (active)
       for (int e = p; e < NX; e += nl)
         X[e] = P.cell_vertices[cell * NV * dim + e];
     int nb[2 * dim];
@@ -683,14 +683,13 @@ namespace dgops
         // tabulated: LAST-direction speed and JxW on this thread's line,
         // a . n dA on the faces it touches (lower face of this cell, upper
         // face = lower face of the upper neighbour, boundary faces apart)
-        const int64_t tc = cell * (dim + 2) * nd;
+        table_line<dim, n, LAST>(tab_c, cell, LAST, p, active, cv[LAST]);
+        if (fin == fin_integrate && active && P.has_mass)
+          line_jxw<dim, n, LAST, T, L>(X, Bd->xq, Bd->w, p, jxw);
+        else
 #pragma unroll
-        for (int i = 0; i < n; ++i)
-          {
-            const int o = L::template off<LAST>(p, i);
-            cv[LAST][i] = active ? tab_c[tc + LAST * nd + o] : (T)0;
-            jxw[i] = (active && P.has_mass) ? tab_c[tc + dim * nd + o] : (T)1;
-          }
+          for (int i = 0; i < n; ++i)
+            jxw[i] = 1;
 #pragma unroll
         for (int d = 0; d < dim; ++d)
           {
@@ -765,7 +764,7 @@ namespace dgops
 #pragma unroll
           for (int i = 0; i < n; ++i)
             acc[i] = 0;
-          if (P.has_mass)
+          if (fin == fin_integrate && P.has_mass)
             {
 #pragma unroll
               for (int b = 0; b < NC; ++b)
@@ -920,8 +919,17 @@ namespace dgops
                 __syncthreads();
                 lds_read_line<dim, n, 0>(g0_l, p, ij);
               }
+            else if (active)
+              {
+                line_jxw<dim, n, 0, T, L>(X, Bd->xq, Bd->w, p, ij);
+#pragma unroll
+                for (int i = 0; i < n; ++i)
+                  ij[i] = (T)1 / ij[i];
+              }
             else
-              table_line<dim, n, 0>(tab_c, cell, dim + 1, p, active, ij);
+#pragma unroll
+              for (int i = 0; i < n; ++i)
+                ij[i] = 1;
 #pragma unroll
             for (int i = 0; i < n; ++i)
               acc[i] *= ij[i];
