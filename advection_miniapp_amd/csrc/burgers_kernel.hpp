@@ -72,7 +72,7 @@ namespace dgops
     __shared__ double fo[dim][mf], fn[dim][mf];
     __shared__ double X[NV * dim];
 
-    const int64_t cell = blockIdx.x;
+    const int64_t cell = xcd_block(blockIdx.x, gridDim.x);
     const int     tid  = threadIdx.x;
     const int     nvec = dim * nd;
     const T      *src  = io.src + cell * nvec;

@@ -9,13 +9,20 @@
 //   3. pointwise inverse on the real-pair blocks           :1019-1052
 //   4. d sweeps with V                                     :1055-1065
 //   5. stage transform T                                   :1068-1110
-// Per cell the 1D eigenbasis of each direction is the one of the sign of
-// the cell's mean reference-space velocity (:838-844).
+// The reference keeps one 1D eigenbasis per velocity sign and picks the one
+// of the sign of the cell's mean reference-space velocity (:838-844). The
+// upwind matrices of the two signs are mirror images,
+//   M^-1 K(-) = -R M^-1 K(+) R,   R M R = M,   R = line reversal,
+// so a direction with a < 0 uses the eigenbasis of the + sign on the
+// reversed line and the eigenvalues of a M^-1 K(-) are |a| lambda(+). The
+// reversal is a permutation of the line's points: it is folded into the
+// per-lane addresses of the loads, LDS accesses and stores, and the 1D
+// tables stay uniform across the wavefront.
 //
 // Step 3 diagonalizes every 2x2 real-pair block [[a, b], [-b, a]] with
 // U = [[1, 1], [i, -i]] (eigenvalues lambda, conj(lambda)) and divides by
-// sigma + sum_d a_d lambda_d in complex arithmetic; the same holds for the
-// stage pairs of A^-1. Lines are owned by threads as in op_kernel.hpp.
+// sigma + sum_d |a_d| lambda_d in complex arithmetic; the same holds for
+// the stage pairs of A^-1. Lines are owned by threads as in op_kernel.hpp.
 #pragma once
 
 #include "op_kernel.hpp"
@@ -25,9 +32,10 @@ namespace dgops
   template <typename T>
   struct FdmDev
   {
-    T   VinvMinv[2][max_n][max_n]; // [sign][i][j]
-    T   V[2][max_n][max_n];
-    T   lre[2][max_n], lim[2][max_n];
+    // 1D eigenbasis of M^-1 K(+) (velocity sign +)
+    T   VinvMinv[max_n][max_n]; // [i][j]
+    T   V[max_n][max_n];
+    T   lre[max_n], lim[max_n];
     int kind[max_n];               // 0 real, 1 first of pair, 2 second
     int n_blocks;                  // blocks per direction
     int block_col[max_n];          // first column of block b
@@ -50,51 +58,61 @@ namespace dgops
     int64_t       block;         // stage block size (dofs)
   };
 
-  template <int dim, int n, typename T>
-  __device__ __forceinline__ void
-  fdm_block(T *data, const FdmDev<T> &F, const int (&sg)[3],
-            const T (&abar)[3], T invdet, T inv_dt, int sb, int bx, int by,
-            int bz)
+  // point i of a line, read or written mirrored when `rev`
+  template <int n>
+  __device__ __forceinline__ int
+  mirror(int i, bool rev)
   {
-    constexpr int STR = Lay<dim, n>::stride;
-    // factors: 0 stage, 1 x, 2 y, 3 z
-    int fcol[4], fsize[4];
-    fcol[0]  = F.stage_block_col[sb];
-    fsize[0] = F.stage_block_size[sb];
-    const int bb[3] = {bx, by, bz};
-    for (int d = 0; d < 3; ++d)
+    return rev ? n - 1 - i : i;
+  }
+
+  template <int dim, int n, int dir, typename T>
+  __device__ __forceinline__ void
+  lds_read_line_m(const T *buf, int p, bool rev, T (&v)[n])
+  {
+#pragma unroll
+    for (int i = 0; i < n; ++i)
+      v[i] = buf[Lay<dim, n>::template off<dir>(p, mirror<n>(i, rev))];
+  }
+
+  template <int dim, int n, int dir, typename T>
+  __device__ __forceinline__ void
+  lds_write_line_m(T *buf, int p, bool rev, const T (&v)[n])
+  {
+#pragma unroll
+    for (int i = 0; i < n; ++i)
+      buf[Lay<dim, n>::template off<dir>(p, mirror<n>(i, rev))] = v[i];
+  }
+
+  // inverse of one block of 2^NB real-pair-coupled points (NB paired
+  // factors among stage, x, y, z): load, U^-1 on every paired factor,
+  // complex division by the eigenvalue sum, U, store. `delta[b]` is the
+  // storage distance to the partner along paired factor b, lr the real
+  // part of the eigenvalue sum and li[b] the imaginary part of paired
+  // factor b (conjugated when the point's bit b is set).
+  template <int NB, typename T>
+  __device__ __forceinline__ void
+  fdm_block(T *data, int base, const int (&delta)[4], T lr,
+            const T (&li)[4], T invdet)
+  {
+    constexpr int NE = 1 << NB;
+    T             re[NE], im[NE];
+#pragma unroll
+    for (int e = 0; e < NE; ++e)
       {
-        if (d < dim)
-          {
-            fcol[1 + d]  = F.block_col[bb[d]];
-            fsize[1 + d] = F.block_size[bb[d]];
-          }
-        else
-          {
-            fcol[1 + d]  = 0;
-            fsize[1 + d] = 1;
-          }
-      }
-    int bitf[4], nbits = 0;
-    for (int f = 0; f < 4; ++f)
-      if (fsize[f] == 2)
-        bitf[nbits++] = f;
-    const int ne = 1 << nbits;
-    T         re[16], im[16];
-    for (int e = 0; e < ne; ++e)
-      {
-        int off[4] = {0, 0, 0, 0};
-        for (int b = 0; b < nbits; ++b)
-          off[bitf[b]] = (e >> b) & 1;
-        const int js = fcol[0] + off[0];
-        const int ix = fcol[1] + off[1], iy = fcol[2] + off[2],
-                  iz = fcol[3] + off[3];
-        re[e] = data[js * STR + ix + n * (iy + n * iz)];
+        int a = base;
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+          if ((e >> b) & 1)
+            a += delta[b];
+        re[e] = data[a];
         im[e] = 0;
       }
     // U^-1 on every pair bit: (x0, x1) -> (1/2 (x0 - i x1), 1/2 (x0 + i x1))
-    for (int b = 0; b < nbits; ++b)
-      for (int e = 0; e < ne; ++e)
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int e = 0; e < NE; ++e)
         if (!((e >> b) & 1))
           {
             const int e1 = e | (1 << b);
@@ -104,33 +122,24 @@ namespace dgops
             re[e1]       = (T)0.5 * (r0 - i1);
             im[e1]       = (T)0.5 * (i0 + r1);
           }
-    // divide by sigma + sum_d a_d lambda_d (bit 1: conjugate eigenvalue)
-    for (int e = 0; e < ne; ++e)
+#pragma unroll
+    for (int e = 0; e < NE; ++e)
       {
-        int off[4] = {0, 0, 0, 0};
-        for (int b = 0; b < nbits; ++b)
-          off[bitf[b]] = (e >> b) & 1;
-        const int js = fcol[0];
-        T         lr = F.mre[js] * inv_dt;
-        T         li = (fsize[0] == 2 ? (off[0] ? -F.mim[js] : F.mim[js]) :
-                                        (T)0) *
-               inv_dt;
-        for (int d = 0; d < dim; ++d)
-          {
-            const int c = fcol[1 + d];
-            lr += abar[d] * F.lre[sg[d]][c];
-            if (fsize[1 + d] == 2)
-              li += abar[d] * (off[1 + d] ? -F.lim[sg[d]][c] : F.lim[sg[d]][c]);
-          }
-        const T den = lr * lr + li * li;
-        const T sr = invdet * lr / den, si = -invdet * li / den;
+        T lim = 0;
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+          lim += ((e >> b) & 1) ? -li[b] : li[b];
+        const T den = lr * lr + lim * lim;
+        const T sr = invdet * lr / den, si = -invdet * lim / den;
         const T r = re[e], i = im[e];
         re[e]     = r * sr - i * si;
         im[e]     = r * si + i * sr;
       }
     // U on every pair bit: (y+, y-) -> (y+ + y-, i (y+ - y-))
-    for (int b = 0; b < nbits; ++b)
-      for (int e = 0; e < ne; ++e)
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int e = 0; e < NE; ++e)
         if (!((e >> b) & 1))
           {
             const int e1 = e | (1 << b);
@@ -140,15 +149,15 @@ namespace dgops
             re[e1]       = -(i0 - i1);
             im[e1]       = r0 - r1;
           }
-    for (int e = 0; e < ne; ++e)
+#pragma unroll
+    for (int e = 0; e < NE; ++e)
       {
-        int off[4] = {0, 0, 0, 0};
-        for (int b = 0; b < nbits; ++b)
-          off[bitf[b]] = (e >> b) & 1;
-        const int js = fcol[0] + off[0];
-        const int ix = fcol[1] + off[1], iy = fcol[2] + off[2],
-                  iz = fcol[3] + off[3];
-        data[js * STR + ix + n * (iy + n * iz)] = re[e];
+        int a = base;
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+          if ((e >> b) & 1)
+            a += delta[b];
+        data[a] = re[e];
       }
   }
 
@@ -171,115 +180,162 @@ namespace dgops
     const int         tid  = threadIdx.x;
     const int         lc   = tid / nl;
     const int         p    = tid - lc * nl;
-    const int64_t     cell = (int64_t)blockIdx.x * CPB + lc;
+    const int64_t     cell =
+      (int64_t)xcd_block(blockIdx.x, gridDim.x) * CPB + lc;
     const bool        active = (lc < CPB) && (cell < P.n_cells);
     const int         lcs    = lc < CPB ? lc : 0;
     Tc               *data   = s_data + lcs * NS * STR;
     Tc               *tb     = s_t + lcs * STR;
 
-    int sg[3] = {0, 0, 0};
-    Tc  abar[3] = {0, 0, 0};
-    Tc  invdet  = 1;
+    bool rv[3]   = {false, false, false};
+    Tc   aabs[3] = {0, 0, 0};
+    Tc   invdet  = 1;
     if (active)
       {
         for (int d = 0; d < dim; ++d)
           {
             const double a = P.cell_velocity[cell * dim + d] * P.vel_scale;
-            abar[d]        = (Tc)a;
-            sg[d]          = a < 0. ? 1 : 0;
+            aabs[d]        = (Tc)fabs(a);
+            rv[d]          = a < 0.;
           }
         invdet = (Tc)P.cell_invdet[cell];
       }
 
-    // 1-2: load, stage transform, V^-1 M^-1 sweeps x, y, (z)
-    Tc w[NS][n];
+    // 1-2: load (x mirrored), stage transform, V^-1 M^-1 sweeps x, y, (z)
+    // with the y and z mirrors on the LDS reads
     {
       Tc r[NS][n];
 #pragma unroll
       for (int t = 0; t < NS; ++t)
 #pragma unroll
         for (int i = 0; i < n; ++i)
-          r[t][i] = active ? (Tc)src[t * P.block + cell * nd +
-                                     L::template off<0>(p, i)] :
-                             (Tc)0;
+          r[t][i] =
+            active ? (Tc)src[t * P.block + cell * nd +
+                             L::template off<0>(p, mirror<n>(i, rv[0]))] :
+                     (Tc)0;
 #pragma unroll
       for (int j = 0; j < NS; ++j)
+        {
+          Tc w[n], t[n];
 #pragma unroll
-        for (int i = 0; i < n; ++i)
-          {
-            Tc s = 0;
+          for (int i = 0; i < n; ++i)
+            {
+              Tc s = 0;
 #pragma unroll
-            for (int t = 0; t < NS; ++t)
-              s = fma(F.Tinv[j][t], r[t][i], s);
-            w[j][i] = NS == 1 ? r[0][i] : s;
-          }
+              for (int k = 0; k < NS; ++k)
+                s = fma(F.Tinv[j][k], r[k][i], s);
+              w[i] = NS == 1 ? r[0][i] : s;
+            }
+          mat_apply<n>(F.VinvMinv, w, t);
+          lds_write_line<dim, n, 0>(tb, p, t);
+          __syncthreads();
+          lds_read_line_m<dim, n, 1>(tb, p, rv[1], t);
+          mat_apply<n>(F.VinvMinv, t, w);
+          if constexpr (dim == 3)
+            {
+              lds_write_line<dim, n, 1>(tb, p, w);
+              __syncthreads();
+              lds_read_line_m<dim, n, 2>(tb, p, rv[2], t);
+              mat_apply<n>(F.VinvMinv, t, w);
+            }
+          lds_write_line<dim, n, LAST>(data + j * STR, p, w);
+          __syncthreads();
+        }
     }
-#pragma unroll
-    for (int j = 0; j < NS; ++j)
-      {
-        Tc t[n];
-        mat_apply<n>(F.VinvMinv[sg[0]], w[j], t);
-        lds_write_line<dim, n, 0>(tb, p, t);
-        __syncthreads();
-        lds_read_line<dim, n, 1>(tb, p, t);
-        mat_apply<n>(F.VinvMinv[sg[1]], t, w[j]);
-        if constexpr (dim == 3)
-          {
-            __syncthreads();
-            lds_write_line<dim, n, 1>(tb, p, w[j]);
-            __syncthreads();
-            lds_read_line<dim, n, 2>(tb, p, t);
-            mat_apply<n>(F.VinvMinv[sg[2]], t, w[j]);
-          }
-        lds_write_line<dim, n, LAST>(data + j * STR, p, w[j]);
-        __syncthreads();
-      }
 
     // 3: pointwise inverse on the blocks of this cell
     {
-      const int nbd  = F.n_blocks;
-      const int nspb = dim == 3 ? nbd * nbd * nbd : nbd * nbd;
+      const int nbd   = F.n_blocks;
+      const int nspb  = dim == 3 ? nbd * nbd * nbd : nbd * nbd;
       const int ntask = F.n_stage_blocks * nspb;
+      const Tc  idt   = (Tc)P.inv_dt;
       if (active)
         for (int task = p; task < ntask; task += nl)
           {
             const int sb = task / nspb;
             int       rb = task - sb * nspb;
-            const int bx = rb % nbd;
+            int       bd[3];
+            bd[0] = rb % nbd;
             rb /= nbd;
-            const int by = rb % nbd;
-            const int bz = dim == 3 ? rb / nbd : 0;
-            fdm_block<dim, n, Tc>(data, F, sg, abar, invdet, (Tc)P.inv_dt, sb,
-                                  bx, by, bz);
+            bd[1] = rb % nbd;
+            bd[2] = dim == 3 ? rb / nbd : 0;
+            // factor 0: stage, 1..dim: directions
+            const int js   = F.stage_block_col[sb];
+            int       base = js * STR;
+            Tc        lr   = F.mre[js] * idt;
+            int       delta[4] = {0, 0, 0, 0};
+            Tc        li[4]    = {0, 0, 0, 0};
+            int       nb       = 0;
+            if (F.stage_block_size[sb] == 2)
+              {
+                delta[0] = STR;
+                li[0]    = F.mim[js] * idt;
+                nb       = 1;
+              }
+            int stride = 1;
+#pragma unroll
+            for (int d = 0; d < dim; ++d)
+              {
+                const int c = F.block_col[bd[d]];
+                base += c * stride;
+                lr += aabs[d] * F.lre[c];
+                if (F.block_size[bd[d]] == 2)
+                  {
+                    // nb < 4 here: at most one stage bit and dim bits
+                    delta[nb] = stride;
+                    li[nb]    = aabs[d] * F.lim[c];
+                    ++nb;
+                  }
+                stride *= n;
+              }
+            switch (nb)
+              {
+                case 0:
+                  fdm_block<0, Tc>(data, base, delta, lr, li, invdet);
+                  break;
+                case 1:
+                  fdm_block<1, Tc>(data, base, delta, lr, li, invdet);
+                  break;
+                case 2:
+                  fdm_block<2, Tc>(data, base, delta, lr, li, invdet);
+                  break;
+                case 3:
+                  fdm_block<3, Tc>(data, base, delta, lr, li, invdet);
+                  break;
+                default:
+                  fdm_block<4, Tc>(data, base, delta, lr, li, invdet);
+                  break;
+              }
           }
       __syncthreads();
     }
 
-    // 4: V sweeps LAST, ..., x
+    // 4: V sweeps LAST, ..., x with the mirrors on the writes; the x
+    // mirror goes into the store
+    Tc w[NS][n];
 #pragma unroll
     for (int j = 0; j < NS; ++j)
       {
         Tc t[n];
         lds_read_line<dim, n, LAST>(data + j * STR, p, t);
-        mat_apply<n>(F.V[sg[LAST]], t, w[j]);
+        mat_apply<n>(F.V, t, w[j]);
         if constexpr (dim == 3)
           {
-            lds_write_line<dim, n, 2>(tb, p, w[j]);
+            lds_write_line_m<dim, n, 2>(tb, p, rv[2], w[j]);
             __syncthreads();
             lds_read_line<dim, n, 1>(tb, p, t);
-            mat_apply<n>(F.V[sg[1]], t, w[j]);
-            __syncthreads();
-            lds_write_line<dim, n, 1>(tb, p, w[j]);
+            mat_apply<n>(F.V, t, w[j]);
+            lds_write_line_m<dim, n, 1>(tb, p, rv[1], w[j]);
           }
         else
-          lds_write_line<dim, n, 1>(tb, p, w[j]);
+          lds_write_line_m<dim, n, 1>(tb, p, rv[1], w[j]);
         __syncthreads();
         lds_read_line<dim, n, 0>(tb, p, t);
-        mat_apply<n>(F.V[sg[0]], t, w[j]);
+        mat_apply<n>(F.V, t, w[j]);
         __syncthreads();
       }
 
-    // 5: back stage transform and store (x-lines)
+    // 5: back stage transform and store (x-lines, mirrored in x)
     if (active)
 #pragma unroll
       for (int t = 0; t < NS; ++t)
@@ -290,7 +346,8 @@ namespace dgops
 #pragma unroll
             for (int j = 0; j < NS; ++j)
               s = fma(F.Tm[t][j], w[j][i], s);
-            dst[t * P.block + cell * nd + L::template off<0>(p, i)] =
+            dst[t * P.block + cell * nd +
+                L::template off<0>(p, mirror<n>(i, rv[0]))] =
               (Tout)(NS == 1 ? w[0][i] : s);
           }
   }

@@ -131,27 +131,26 @@ namespace dgops
 
     template <typename T>
     void
-    fill_spatial(FdmDev<T> &F, const RealPairEig (&E)[2],
-                 const std::vector<double> (&VinvMinv)[2], int n)
+    fill_spatial(FdmDev<T> &F, const RealPairEig &E,
+                 const std::vector<double> &VinvMinv, int n)
     {
-      for (int c = 0; c < 2; ++c)
-        for (int i = 0; i < n; ++i)
-          {
-            for (int j = 0; j < n; ++j)
-              {
-                F.VinvMinv[c][i][j] = (T)VinvMinv[c][i * n + j];
-                F.V[c][i][j]        = (T)E[c].V[i * n + j];
-              }
-            F.lre[c][i] = (T)E[c].lam_re[i];
-            F.lim[c][i] = (T)E[c].lam_im[i];
-          }
+      for (int i = 0; i < n; ++i)
+        {
+          for (int j = 0; j < n; ++j)
+            {
+              F.VinvMinv[i][j] = (T)VinvMinv[i * n + j];
+              F.V[i][j]        = (T)E.V[i * n + j];
+            }
+          F.lre[i] = (T)E.lam_re[i];
+          F.lim[i] = (T)E.lam_im[i];
+        }
       int nb = 0;
       for (int i = 0; i < n;)
         {
-          F.kind[i]          = E[0].kind[i];
+          F.kind[i]          = E.kind[i];
           F.block_col[nb]    = i;
-          F.block_size[nb]   = E[0].kind[i] == 1 ? 2 : 1;
-          if (E[0].kind[i] == 1)
+          F.block_size[nb]   = E.kind[i] == 1 ? 2 : 1;
+          if (E.kind[i] == 1)
             F.kind[i + 1] = 2;
           i += F.block_size[nb];
           ++nb;
@@ -195,8 +194,9 @@ namespace dgops
       F.n_stage_blocks = nb;
     }
 
-    // 1D upwind-DG matrices and their real-pair eigendecomposition for both
-    // velocity signs (advection_solver_irk_real.cc:1490-1541), cell data
+    // 1D upwind-DG matrices of the + velocity sign and their real-pair
+    // eigendecomposition (advection_solver_irk_real.cc:1490-1541); the
+    // - sign is its mirror image (see fdm_kernel.hpp), cell data
     void
     setup_fdm(Operator &op)
     {
@@ -204,39 +204,30 @@ namespace dgops
         return;
       const int      n = op.n;
       const Basis1D &B = op.basis;
-      RealPairEig    E[2];
-      std::vector<double> VinvMinv[2];
-      for (int c = 0; c < 2; ++c)
-        {
-          std::vector<double> K(n * n, 0.), M(n * n, 0.);
-          for (int q = 0; q < n; ++q)
-            for (int i = 0; i < n; ++i)
-              for (int j = 0; j < n; ++j)
-                {
-                  K[i * n + j] -= B.G[q][i] * B.S[q][j] * B.gauss_wts[q];
-                  M[i * n + j] += B.S[q][i] * B.S[q][j] * B.gauss_wts[q];
-                }
-          const double sign  = c == 0 ? 1. : -1.;
-          const double alpha = op.cfg.flux_alpha;
-          // nodal basis at 0 and 1: phi_i(0) = delta_i0, phi_i(1) = delta_i,n-1
-          K[0] += -(0.5 - alpha * 0.5 * sign);
-          K[(n - 1) * n + (n - 1)] += (0.5 + alpha * 0.5 * sign);
-          const auto Minv = invert_dense(M, n);
-          std::vector<double> D(n * n, 0.);
-          for (int i = 0; i < n; ++i)
-            for (int j = 0; j < n; ++j)
-              for (int k = 0; k < n; ++k)
-                D[i * n + j] += Minv[i * n + k] * K[k * n + j];
-          E[c] = real_pair_eigen(D, n);
-          VinvMinv[c].assign(n * n, 0.);
-          for (int i = 0; i < n; ++i)
-            for (int j = 0; j < n; ++j)
-              for (int k = 0; k < n; ++k)
-                VinvMinv[c][i * n + j] += E[c].Vinv[i * n + k] * Minv[k * n + j];
-        }
-      require(E[0].kind == E[1].kind,
-              "FDM: both velocity signs need the same eigenvalue pairing",
-              DG_ERR_UNSUPPORTED);
+      std::vector<double> K(n * n, 0.), M(n * n, 0.);
+      for (int q = 0; q < n; ++q)
+        for (int i = 0; i < n; ++i)
+          for (int j = 0; j < n; ++j)
+            {
+              K[i * n + j] -= B.G[q][i] * B.S[q][j] * B.gauss_wts[q];
+              M[i * n + j] += B.S[q][i] * B.S[q][j] * B.gauss_wts[q];
+            }
+      const double alpha = op.cfg.flux_alpha;
+      // nodal basis at 0 and 1: phi_i(0) = delta_i0, phi_i(1) = delta_i,n-1
+      K[0] += -(0.5 - alpha * 0.5);
+      K[(n - 1) * n + (n - 1)] += (0.5 + alpha * 0.5);
+      const auto Minv = invert_dense(M, n);
+      std::vector<double> D(n * n, 0.);
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+          for (int k = 0; k < n; ++k)
+            D[i * n + j] += Minv[i * n + k] * K[k * n + j];
+      const RealPairEig   E = real_pair_eigen(D, n);
+      std::vector<double> VinvMinv(n * n, 0.);
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+          for (int k = 0; k < n; ++k)
+            VinvMinv[i * n + j] += E.Vinv[i * n + k] * Minv[k * n + j];
       FdmDev<double> fd1 = {}, fdS = {};
       FdmDev<float>  ff1 = {}, ffS = {};
       fill_spatial(fd1, E, VinvMinv, n);

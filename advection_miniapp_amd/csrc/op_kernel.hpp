@@ -108,6 +108,19 @@ namespace dgops
     }
   };
 
+  // Workgroups are handed round-robin to the 8 XCDs of the chip, each with
+  // its own L2. This bijection of [0, nb) gives every XCD one contiguous
+  // range of blocks, so the neighbour cells of a Morton-ordered mesh (and
+  // their face traces and tables) are read through the same L2.
+  __device__ __forceinline__ unsigned
+  xcd_block(unsigned b, unsigned nb)
+  {
+    constexpr unsigned nx   = 8;
+    const unsigned     full = nb / nx, rem = nb % nx;
+    const unsigned     x = b % nx, i = b / nx;
+    return x < rem ? x * (full + 1) + i : rem * (full + 1) + (x - rem) * full + i;
+  }
+
   template <int dim, int n>
   constexpr int
   cells_per_block()
@@ -491,7 +504,8 @@ namespace dgops
     const int          tid  = threadIdx.x;
     const int          lc   = tid / nl;
     const int          p    = tid - lc * nl;
-    const int64_t      cell = (int64_t)blockIdx.x * CPB + lc;
+    const int64_t      cell =
+      (int64_t)xcd_block(blockIdx.x, gridDim.x) * CPB + lc;
     const bool         active = (lc < CPB) && (cell < P.n_cells);
     const int          lcs    = lc < CPB ? lc : 0;
 
