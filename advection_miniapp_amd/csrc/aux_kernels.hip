@@ -736,8 +736,8 @@ namespace dgops
         Lay<dim, n>::template idx<dim - 1>(p, 0, q);
     }
 
-    // one block per cell: w adj(J) a at the Gauss points (lexicographic),
-    // and w (adj(J) a)_d on the lower face of every
+    // one block per cell: w adj(J) a and 1/JxW at the Gauss points
+    // (lexicographic), and w (adj(J) a)_d on the lower face of every
     // direction, indexed by the line p of direction d that ends on it
     template <int dim, int n>
     __global__ void
@@ -752,7 +752,7 @@ namespace dgops
       for (int e = threadIdx.x; e < NV * dim; e += blockDim.x)
         X[e] = P.cell_vertices[cell * NV * dim + e];
       __syncthreads();
-      double *tc = tab_cell + cell * dim * nd;
+      double *tc = tab_cell + cell * (dim + 1) * nd;
       for (int pt = threadIdx.x; pt < nd; pt += blockDim.x)
         {
           const int q[3] = {pt % n, (pt / n) % n,
@@ -761,6 +761,7 @@ namespace dgops
           cell_point_data<dim, false>(P, X, B->xq, B->w, q, cv, jxw);
           for (int d = 0; d < dim; ++d)
             tc[d * nd + pt] = cv[d];
+          tc[dim * nd + pt] = 1. / jxw;
         }
       for (int t = threadIdx.x; t < dim * nl; t += blockDim.x)
         {
@@ -933,7 +934,7 @@ namespace dgops
                                 hipMemcpyHostToDevice),
                       "upload boundary faces");
           }
-        op.d_tab_cell[0].alloc(sizeof(double) * dim * op.n_dofs);
+        op.d_tab_cell[0].alloc(sizeof(double) * (dim + 1) * op.n_dofs);
         op.d_tab_face[0].alloc(sizeof(double) * dim * nl *
                                std::max<int64_t>(1, nc));
         op.d_tab_bnd.alloc(sizeof(double) * (1 + dim) * nl *
@@ -957,7 +958,7 @@ namespace dgops
       }
     if (pi == 1)
       {
-        const int64_t ncell = dim * op.n_dofs;
+        const int64_t ncell = (dim + 1) * op.n_dofs;
         const int64_t nface = (int64_t)dim * nl * nc;
         op.d_tab_cell[1].alloc(sizeof(float) * std::max<int64_t>(1, ncell));
         op.d_tab_face[1].alloc(sizeof(float) * std::max<int64_t>(1, nface));

@@ -121,12 +121,15 @@ namespace dgops
     return x < rem ? x * (full + 1) + i : rem * (full + 1) + (x - rem) * full + i;
   }
 
+  // cells per workgroup: two wavefronts' worth of lines, so that every
+  // barrier holds back two waves only and more workgroups share a CU
+  // (3D degree 4: 5 cells, 125 of 128 lanes busy)
   template <int dim, int n>
   constexpr int
   cells_per_block()
   {
     constexpr int nl = Lay<dim, n>::nl;
-    const int     c  = 256 / nl;
+    const int     c  = 128 / nl;
     return c < 1 ? 1 : c;
   }
 
@@ -198,15 +201,15 @@ namespace dgops
       v[i] = buf[Lay<dim, n>::template off<dir>(p, i)];
   }
 
-  // row `row` of a cell's speed table [dim][nd], read along line p of
-  // direction dir
+  // row `row` of a cell's geometry table [dim + 1][nd], read along line p
+  // of direction dir
   template <int dim, int n, int dir, typename T>
   __device__ __forceinline__ void
   table_line(const T *tab, int64_t cell, int row, int p, bool active,
              T (&v)[n])
   {
     constexpr int nd   = Lay<dim, n>::nd;
-    const T      *base = tab + (cell * dim + row) * nd;
+    const T      *base = tab + (cell * (dim + 1) + row) * nd;
 #pragma unroll
     for (int i = 0; i < n; ++i)
       v[i] = active ? base[Lay<dim, n>::template off<dir>(p, i)] : (T)0;
@@ -536,7 +539,30 @@ namespace dgops
     for (int c = 0; c < NC; ++c)
       {
         const T *src = io.src + c * io.comp_stride;
-        if constexpr (dim == 3)
+        if constexpr (dim == 3 && !diff)
+          {
+            // stage the neighbours' face nodes in LDS: thread (i, j) loads
+            // node (t1 = i, t2 = j) of every face, one load per face
+            const int i0 = p % n, j0 = p / n;
+#pragma unroll
+            for (int f = 0; f < 6; ++f)
+              {
+                const int d = f / 2, s = f % 2;
+                const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+                T         v  = 0;
+                if (active && nb[f] >= 0)
+                  {
+                    int ijk[3];
+                    ijk[d]  = s ? 0 : n - 1;
+                    ijk[t1] = i0;
+                    ijk[t2] = j0;
+                    v       = src[(int64_t)nb[f] * nd + ijk[0] + n * ijk[1] +
+                            n * n * ijk[2]];
+                  }
+                f_l[f * nl + j0 * n + i0] = v;
+              }
+          }
+        else if constexpr (dim == 3)
           {
             // step 1: thread (aq, j) interpolates the neighbour's face nodes
             // along the first tangential direction
@@ -613,7 +639,23 @@ namespace dgops
         mat_apply<n>(B.S, r, t);
         lds_write_line<dim, n, 0>(acc_l, p, t);
         __syncthreads();
-        if constexpr (dim == 3)
+        T fv[6];
+        if constexpr (dim == 3 && !diff)
+          {
+            // first tangential interpolation of the staged nodes: thread
+            // (aq, j); written back after the next barrier
+            const int aq = p % n, j = p / n;
+#pragma unroll
+            for (int f = 0; f < 6; ++f)
+              {
+                T v = 0;
+#pragma unroll
+                for (int i = 0; i < n; ++i)
+                  v = fma(B.S[aq][i], f_l[f * nl + j * n + i], v);
+                fv[f] = v;
+              }
+          }
+        else if constexpr (dim == 3)
           {
             // step 2 of the face interpolation: thread (aq, bq)
             const int aq = p % n, bq = p / n;
@@ -675,8 +717,29 @@ namespace dgops
         if constexpr (dim == 3)
           {
             __syncthreads();
+            if constexpr (!diff)
+              {
+                const int aq = p % n, j = p / n;
+#pragma unroll
+                for (int f = 0; f < 6; ++f)
+                  f_l[f * nl + j * n + aq] = fv[f];
+              }
             lds_write_line<dim, n, 1>(acc_l, p, r);
             __syncthreads();
+            if constexpr (!diff)
+              {
+                // second tangential interpolation: thread (aq, bq)
+                const int aq = p % n, bq = p / n;
+#pragma unroll
+                for (int f = 0; f < 6; ++f)
+                  {
+                    T v = 0;
+#pragma unroll
+                    for (int jj = 0; jj < n; ++jj)
+                      v = fma(B.S[bq][jj], f_l[f * nl + jj * n + aq], v);
+                    uplus[c][f] = v;
+                  }
+              }
             lds_read_line<dim, n, 2>(acc_l, p, t);
             mat_apply<n>(B.S, t, r);
           }
@@ -766,6 +829,15 @@ namespace dgops
         fan[f] = fan_l[f * nl + fidx];
     }
 
+    // speed rows of the y and x phases, read once for all components and
+    // in flight while the LAST phase computes
+    T cvy[n], cvx[n];
+    if constexpr (!AN)
+      {
+        if constexpr (dim == 3)
+          table_line<dim, n, 1>(tab_c, cell, 1, p, active, cvy);
+        table_line<dim, n, 0>(tab_c, cell, 0, p, active, cvx);
+      }
 #pragma unroll
     for (int c = 0; c < NC; ++c)
       {
@@ -845,13 +917,13 @@ namespace dgops
         // ---- y direction in 3D ----
         if constexpr (dim == 3)
           {
-            T uq[n], g[n], cvd[n];
+            T        uq[n], g[n];
+            const T(&cvd)[n] = cvy;
             lds_read_line<dim, n, 1>(u_l, p, uq);
             if constexpr (AN)
               lds_read_line<dim, n, 1>(g1_l, p, g);
             else
               {
-                table_line<dim, n, 1>(tab_c, cell, 1, p, active, cvd);
 #pragma unroll
                 for (int i = 0; i < n; ++i)
                   g[i] = fcell * cvd[i] * uq[i];
@@ -885,13 +957,13 @@ namespace dgops
 
         // ---- x direction ----
         {
-          T uq[n], g[n], cvd[n];
+          T        uq[n], g[n];
+          const T(&cvd)[n] = cvx;
           lds_read_line<dim, n, 0>(u_l, p, uq);
           if constexpr (AN)
             lds_read_line<dim, n, 0>(g0_l, p, g);
           else
             {
-              table_line<dim, n, 0>(tab_c, cell, 0, p, active, cvd);
 #pragma unroll
               for (int i = 0; i < n; ++i)
                 g[i] = fcell * cvd[i] * uq[i];
@@ -933,17 +1005,8 @@ namespace dgops
                 __syncthreads();
                 lds_read_line<dim, n, 0>(g0_l, p, ij);
               }
-            else if (active)
-              {
-                line_jxw<dim, n, 0, T, L>(X, Bd->xq, Bd->w, p, ij);
-#pragma unroll
-                for (int i = 0; i < n; ++i)
-                  ij[i] = (T)1 / ij[i];
-              }
             else
-#pragma unroll
-              for (int i = 0; i < n; ++i)
-                ij[i] = 1;
+              table_line<dim, n, 0>(tab_c, cell, dim, p, active, ij);
 #pragma unroll
             for (int i = 0; i < n; ++i)
               acc[i] *= ij[i];
