@@ -76,6 +76,21 @@ class DgSolverControl(ctypes.Structure):
     ]
 
 
+class DgServeRequest(ctypes.Structure):
+    _fields_ = [
+        ("degree", ctypes.c_int),
+        ("cells_per_direction", ctypes.c_int),
+        ("velocity", ctypes.c_double * 3),
+        ("center", ctypes.c_double * 3),
+        ("width", ctypes.c_double),
+        ("final_time", ctypes.c_double),
+        ("courant", ctypes.c_double),
+    ]
+
+
+DG_REQUEST_QUEUED, DG_REQUEST_ACTIVE, DG_REQUEST_DONE, DG_REQUEST_UNKNOWN = \
+    range(4)
+
 _vp = ctypes.c_void_p
 _d = ctypes.c_double
 _i = ctypes.c_int
@@ -124,6 +139,19 @@ SIGNATURES = {
     "dg_export_cell_coordinates": (_i, [_vp, ctypes.POINTER(ctypes.c_int32)]),
     "dg_export_neighbors": (_i, [_vp, ctypes.POINTER(ctypes.c_int32)]),
     "dg_export_cell_vertices": (_i, [_vp, _dp]),
+    "dg_serve_request_config": (_i, [ctypes.POINTER(DgServeRequest),
+                                     ctypes.POINTER(DgConfig), _dp,
+                                     ctypes.POINTER(_i)]),
+    "dg_server_create": (_i, [_i64, _vp, ctypes.POINTER(_vp)]),
+    "dg_server_destroy": (_i, [_vp]),
+    "dg_server_submit": (_i, [_vp, ctypes.POINTER(DgServeRequest),
+                              ctypes.POINTER(_i64)]),
+    "dg_server_step": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "dg_server_request_info": (_i, [_vp, _i64, ctypes.POINTER(_i),
+                                    ctypes.POINTER(_i64), ctypes.POINTER(_i),
+                                    ctypes.POINTER(_i)]),
+    "dg_server_take_result": (_i, [_vp, _i64, _vp]),
+    "dg_server_memory": (_i, [_vp, ctypes.POINTER(_i64)]),
     "dg_host_index_map": (_i, [ctypes.POINTER(DgConfig),
                                ctypes.POINTER(ctypes.c_int64), _dp,
                                ctypes.POINTER(ctypes.c_int64)]),

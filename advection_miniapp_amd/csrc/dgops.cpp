@@ -162,6 +162,64 @@ namespace dgops
     return 1.;
   }
 
+  // Three registers rotate so that the stage input is never written by the
+  // fused kernel that reads it; the last stage writes register 0 (the
+  // solution). Stage times c follow advection_solver.cc:924-937.
+  std::vector<LsrkStage>
+  lsrk_plan(int scheme)
+  {
+    std::vector<double> bi, ai;
+    lsrk_coefficients(scheme, bi, ai);
+    const int              ns = (int)bi.size();
+    std::vector<LsrkStage> plan;
+    int                    r_in = 0, s_in = 0;
+    double                 sum_previous_bi = 0;
+    for (int stage = 0; stage < ns; ++stage)
+      {
+        LsrkStage s = {};
+        s.c         = stage == 0 ? 0. : sum_previous_bi + ai[stage - 1];
+        if (stage > 0)
+          sum_previous_bi += bi[stage - 1];
+        s.b    = bi[stage];
+        s.last = stage == ns - 1;
+        s.r_in = r_in;
+        s.s_in = s_in;
+        if (s.last)
+          {
+            s.a     = 0.;
+            s.s_out = 0;
+            s.r_out = -1;
+            plan.push_back(s);
+            break;
+          }
+        s.a = ai[stage];
+        int cand[2], k = 0;
+        for (int b = 0; b < 3; ++b)
+          if (b != r_in)
+            cand[k++] = b;
+        if (stage == ns - 2)
+          {
+            // the next (last) stage reads r_out and writes register 0
+            s.r_out = cand[0] == 0 ? cand[1] : cand[0];
+            s.s_out = cand[0] == s.r_out ? cand[1] : cand[0];
+          }
+        else if (cand[0] == s_in || cand[1] == s_in)
+          {
+            s.s_out = s_in;
+            s.r_out = cand[0] == s_in ? cand[1] : cand[0];
+          }
+        else
+          {
+            s.s_out = cand[0];
+            s.r_out = cand[1];
+          }
+        plan.push_back(s);
+        r_in = s.r_out;
+        s_in = s.s_out;
+      }
+    return plan;
+  }
+
   OpParams
   Operator::base_params(double t, int n_comp) const
   {
@@ -626,54 +684,11 @@ dg_lsrk_time_step(dg_operator *h, int scheme, double t, double dt,
     require(solution && vec_ri && vec_ki, "null vector");
     require(solution != vec_ri && solution != vec_ki && vec_ri != vec_ki,
             "the three registers must be distinct");
-    std::vector<double> bi, ai;
-    lsrk_coefficients(scheme, bi, ai);
-    const int ns = (int)bi.size();
-    // Three registers rotate so that the stage input is never written by
-    // the fused kernel that reads it; the last stage writes `solution`.
     void *buf[3] = {solution, vec_ri, vec_ki};
-    int   r_in = 0, s_in = 0;
-    double sum_previous_bi = 0;
-    for (int stage = 0; stage < ns; ++stage)
-      {
-        const double c_i =
-          stage == 0 ? 0. : sum_previous_bi + ai[stage - 1];
-        if (stage > 0)
-          sum_previous_bi += bi[stage - 1];
-        const double tt   = t + c_i * dt;
-        const bool   last = stage == ns - 1;
-        if (last)
-          {
-            fused_stage(*op, tt, bi[stage] * dt, 0., buf[r_in], buf[s_in],
-                        buf[0], nullptr, true);
-            break;
-          }
-        int cand[2], k = 0;
-        for (int b = 0; b < 3; ++b)
-          if (b != r_in)
-            cand[k++] = b;
-        int s_out, r_out;
-        if (stage == ns - 2)
-          {
-            // the next (last) stage reads r_out and writes buffer 0
-            r_out = cand[0] == 0 ? cand[1] : cand[0];
-            s_out = cand[0] == r_out ? cand[1] : cand[0];
-          }
-        else if (cand[0] == s_in || cand[1] == s_in)
-          {
-            s_out = s_in;
-            r_out = cand[0] == s_in ? cand[1] : cand[0];
-          }
-        else
-          {
-            s_out = cand[0];
-            r_out = cand[1];
-          }
-        fused_stage(*op, tt, bi[stage] * dt, ai[stage] * dt, buf[r_in],
-                    buf[s_in], buf[s_out], buf[r_out], false);
-        r_in = r_out;
-        s_in = s_out;
-      }
+    for (const LsrkStage &s : lsrk_plan(scheme))
+      fused_stage(*op, t + s.c * dt, s.b * dt, s.a * dt, buf[s.r_in],
+                  buf[s.s_in], buf[s.s_out], s.last ? nullptr : buf[s.r_out],
+                  s.last);
   });
 }
 

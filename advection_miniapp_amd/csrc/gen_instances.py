@@ -63,6 +63,11 @@ def burgers_variants():
     return v
 
 
+def batch_variants():
+    # serving loop: batched explicit LSRK stages, 3D, degrees 2..6
+    return [("batch", 3, n) for n in range(3, 8)]
+
+
 def b(x):
     return "true" if x else "false"
 
@@ -70,6 +75,12 @@ def b(x):
 def emit(path, variants):
     lines = ['#include "../registry.hpp"', "", "namespace dgops", "{"]
     for i, var in enumerate(variants):
+        if var[0] == "batch":
+            _, dim, n = var
+            lines.append(
+                f"  static BatchRegistrar s{i}(batch_key({dim}, {n}), "
+                f"&launch_batch<{dim}, {n}>);")
+            continue
         if var[0] == "burgers":
             _, dim, n, m, cart, epi = var
             lines.append(
@@ -105,8 +116,11 @@ def main(n_files=24):
             return v[2] ** (v[1] + 1) * v[3]
         if v[0] == "burgers":
             return v[3] ** (v[1] + 1)
+        if v[0] == "batch":
+            return v[2] ** (v[1] + 1)
         return v[1] ** (v[0] + 1) * v[2] * (2 if v[6] else 1)
-    variants = op_variants() + fdm_variants() + burgers_variants()
+    variants = (op_variants() + fdm_variants() + burgers_variants() +
+                batch_variants())
     # spread heavy (3D, large n) variants evenly
     variants.sort(key=lambda v: -weight(v))
     groups = [[] for _ in range(n_files)]

@@ -143,4 +143,16 @@ namespace dgops
                        const void     *io);
 
   double velocity_time_factor(const dg_config &cfg, double t);
+
+  // one stage of a low-storage Runge-Kutta step on three rotating
+  // registers (0 = solution): stage time t + c dt, factors b dt and a dt,
+  // reads registers r_in (stage input) and s_in (solution), writes s_out
+  // and, unless last, r_out
+  struct LsrkStage
+  {
+    double c, b, a;
+    int    r_in, s_in, s_out, r_out;
+    bool   last;
+  };
+  std::vector<LsrkStage> lsrk_plan(int scheme);
 } // namespace dgops

@@ -124,13 +124,17 @@ namespace dgops
   // cells per workgroup: two wavefronts' worth of lines, so that every
   // barrier holds back two waves only and more workgroups share a CU
   // (3D degree 4: 5 cells, 125 of 128 lanes busy)
+  constexpr int
+  block_cells(int lines_per_cell)
+  {
+    return lines_per_cell >= 128 ? 1 : 128 / lines_per_cell;
+  }
+
   template <int dim, int n>
   constexpr int
   cells_per_block()
   {
-    constexpr int nl = Lay<dim, n>::nl;
-    const int     c  = 128 / nl;
-    return c < 1 ? 1 : c;
+    return block_cells(Lay<dim, n>::nl);
   }
 
   // ---- small register contractions ---------------------------------------
@@ -468,13 +472,17 @@ namespace dgops
   // L_c the advection operator at velocity vel_factor[c] * a(x).
   // With `diff` (axis-aligned cells) L also contains the SIPG diffusion
   // operator of advection_diffusion_solver.cc:531-613.
+  // The operator on the cells of workgroup `block` (one batch of
+  // cells_per_block cells); P and io live in kernel arguments or, for the
+  // batched launch, in global memory (uniform per workgroup either way).
   template <int dim, int n, int NC, typename T, bool cartesian, bool skew,
             bool diff, int fin, int epi>
-  __global__ void __launch_bounds__(256)
-  dg_operator_kernel(const OpParams                       P,
-                     const DevBasis<T> *__restrict__      Bp,
-                     const DevBasis<double> *__restrict__ Bd,
-                     const OpBuffers<T>                   io)
+  __device__ __forceinline__ void
+  operator_body(const OpParams                       &P,
+                const DevBasis<T> *__restrict__      Bp,
+                const DevBasis<double> *__restrict__ Bd,
+                const OpBuffers<T>                   &io,
+                unsigned                              block)
   {
     static_assert(!diff || cartesian, "SIPG needs axis-aligned cells");
     using L            = Lay<dim, n>;
@@ -507,8 +515,7 @@ namespace dgops
     const int          tid  = threadIdx.x;
     const int          lc   = tid / nl;
     const int          p    = tid - lc * nl;
-    const int64_t      cell =
-      (int64_t)xcd_block(blockIdx.x, gridDim.x) * CPB + lc;
+    const int64_t      cell = (int64_t)block * CPB + lc;
     const bool         active = (lc < CPB) && (cell < P.n_cells);
     const int          lcs    = lc < CPB ? lc : 0;
 
@@ -1066,5 +1073,49 @@ namespace dgops
         if (NC > 1)
           __syncthreads();
       }
+  }
+
+  template <int dim, int n, int NC, typename T, bool cartesian, bool skew,
+            bool diff, int fin, int epi>
+  __global__ void __launch_bounds__(256)
+  dg_operator_kernel(const OpParams                       P,
+                     const DevBasis<T> *__restrict__      Bp,
+                     const DevBasis<double> *__restrict__ Bd,
+                     const OpBuffers<T>                   io)
+  {
+    operator_body<dim, n, NC, T, cartesian, skew, diff, fin, epi>(
+      P, Bp, Bd, io, xcd_block(blockIdx.x, gridDim.x));
+  }
+
+  // one member of a batched launch: an independent operator (its own mesh,
+  // speed and vectors) of the same variant, owning the workgroups
+  // [first_block, first_block + blocks)
+  template <typename T>
+  struct BatchItem
+  {
+    OpParams     P;
+    OpBuffers<T> io;
+    unsigned     first_block;
+  };
+
+  // Several operators of one variant in a single launch (the serving loop
+  // advances all active solves of one degree by one LSRK stage with it).
+  // Items are sorted by first_block; the lookup is uniform per workgroup.
+  template <int dim, int n, int NC, typename T, bool cartesian, bool skew,
+            bool diff, int fin, int epi>
+  __global__ void __launch_bounds__(256)
+  dg_operator_batch_kernel(const BatchItem<T> *__restrict__      items,
+                           int                                  n_items,
+                           const DevBasis<T> *__restrict__      Bp,
+                           const DevBasis<double> *__restrict__ Bd)
+  {
+    const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+    int            r = 0;
+    for (int k = 1; k < n_items; ++k)
+      if (items[k].first_block <= b)
+        r = k;
+    const BatchItem<T> &it = items[r];
+    operator_body<dim, n, NC, T, cartesian, skew, diff, fin, epi>(
+      it.P, Bp, Bd, it.io, b - it.first_block);
   }
 } // namespace dgops

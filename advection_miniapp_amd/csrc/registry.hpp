@@ -130,6 +130,44 @@ namespace dgops
     return hipGetLastError();
   }
 
+  // batched explicit LSRK stages of independent axis-aligned,
+  // constant-speed operators of one degree (serving loop)
+  using BatchLaunch = hipError_t (*)(const void *items, int n_items,
+                                     unsigned blocks, const void *basis,
+                                     hipStream_t stream);
+
+  inline uint32_t
+  batch_key(int dim, int n)
+  {
+    return (uint32_t)dim | (uint32_t)n << 2 | 1u << 21;
+  }
+
+  std::map<uint32_t, BatchLaunch> &batch_registry();
+
+  struct BatchRegistrar
+  {
+    BatchRegistrar(uint32_t key, BatchLaunch f) { batch_registry()[key] = f; }
+  };
+
+  template <int dim, int n>
+  hipError_t
+  launch_batch(const void *items, int n_items, unsigned blocks,
+               const void *basis, hipStream_t stream)
+  {
+    constexpr int CPB = cells_per_block<dim, n>();
+    constexpr int nl  = Lay<dim, n>::nl;
+    if (blocks == 0 || n_items <= 0)
+      return hipSuccess;
+    hipLaunchKernelGGL((dg_operator_batch_kernel<dim, n, 1, double, true,
+                                                 false, false,
+                                                 fin_inverse_mass, epi_lsrk>),
+                       dim3(blocks), dim3(CPB * nl), 0, stream,
+                       (const BatchItem<double> *)items, n_items,
+                       (const DevBasis<double> *)basis,
+                       (const DevBasis<double> *)basis);
+    return hipGetLastError();
+  }
+
   template <int dim, int n, int NC, typename T, bool cart, bool skew,
             bool diff, int fin, int epi>
   hipError_t

@@ -169,6 +169,98 @@ def time_irk(args):
                 speedup=out["fp64"]["ms"] / out[best]["ms"], **out)
 
 
+def serving_requests(n_req, final_time, seed=2024, rate=200.0):
+    """The serving workload: Poisson arrivals (mean `rate` per second),
+    degree uniform in 2..6, cells per direction from {8, 16, 24, 32, 48},
+    unit velocity uniform on the sphere, Gaussian with random centre and
+    width uniform in [0.05, 0.2], end time `final_time`."""
+    import numpy as np
+    from advection_miniapp_amd.serving import serve_request
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    reqs = []
+    for _ in range(n_req):
+        a = rng.standard_normal(3)
+        reqs.append(serve_request(int(rng.integers(2, 7)),
+                                  int(rng.choice([8, 16, 24, 32, 48])),
+                                  a / np.linalg.norm(a),
+                                  rng.uniform(0.2, 0.8, 3),
+                                  rng.uniform(0.05, 0.2),
+                                  final_time=final_time, courant=0.3))
+    return arrivals, reqs
+
+
+def time_serving(args):
+    """Continuous batching (Server) against serial one-at-a-time solves of
+    the same arrival stream; latency = completion - arrival."""
+    import numpy as np
+    import torch
+    from advection_miniapp_amd import (AdvectionOperation,
+                                       LowStorageRungeKuttaIntegrator)
+    from advection_miniapp_amd import _lib as L
+    from advection_miniapp_amd.serving import Server, request_config
+    arrivals, reqs = serving_requests(args.serve_requests,
+                                      args.serve_final_time)
+    work = []
+    for r in reqs:
+        cfg, dt, n = request_config(r)
+        nd = (r.cells_per_direction * (r.degree + 1)) ** 3
+        work.append(nd * n)
+    # batched server in wall-clock time
+    srv = Server(int(args.serve_budget_gb * (1 << 30)))
+    ids, done = {}, {}
+    nxt, t0 = 0, time.perf_counter()
+    while len(done) < len(reqs):
+        now = time.perf_counter() - t0
+        while nxt < len(reqs) and arrivals[nxt] <= now:
+            ids[srv.submit(reqs[nxt])] = nxt
+            nxt += 1
+        if len(ids) == len(done):
+            time.sleep(max(0.0, arrivals[nxt] - now))
+            continue
+        srv.step()
+        now = time.perf_counter() - t0
+        for rid, j in ids.items():
+            if j not in done and srv.info(rid)["state"] == L.DG_REQUEST_DONE:
+                done[j] = now
+                srv.take_result(rid)
+    lat_b = np.array([done[j] - arrivals[j] for j in range(len(reqs))])
+    span_b = max(done.values()) - arrivals[0]
+    del srv
+    # serial: one solve at a time, in arrival order
+    integ = LowStorageRungeKuttaIntegrator()
+    finish, lat_s = 0.0, []
+    for j, r in enumerate(reqs):
+        cfg, dt, n = request_config(r)
+        op = AdvectionOperation(cfg)
+        u = op.initialize_dof_vector()
+        r1, r2 = op.initialize_dof_vector(), op.initialize_dof_vector()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        op.project_initial(u, 0.0)
+        for s in range(n):
+            integ.perform_time_step(op, s * dt, dt, u, r1, r2)
+        torch.cuda.synchronize()
+        dur = time.perf_counter() - t1
+        finish = max(finish, arrivals[j]) + dur
+        lat_s.append(finish - arrivals[j])
+        del op, u, r1, r2
+    lat_s = np.array(lat_s)
+    span_s = finish - arrivals[0]
+    tot = float(sum(work))
+    return dict(
+        workload=f"{len(reqs)} requests, Poisson 200/s (seed 2024), degree "
+                 "2..6, 8..48 cells per direction, T = "
+                 f"{args.serve_final_time}, LSRK(5,4) at Courant 0.3",
+        dof_steps=tot,
+        batched=dict(p50_s=float(np.percentile(lat_b, 50)),
+                     p99_s=float(np.percentile(lat_b, 99)),
+                     makespan_s=span_b, dof_per_s=5 * tot / span_b),
+        serial=dict(p50_s=float(np.percentile(lat_s, 50)),
+                    p99_s=float(np.percentile(lat_s, 99)),
+                    makespan_s=span_s, dof_per_s=5 * tot / span_s))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +271,11 @@ def main():
     ap.add_argument("--irk-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-irk", action="store_true")
+    ap.add_argument("--serving", action="store_true",
+                    help="also run the serving workload (tens of seconds)")
+    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-final-time", type=float, default=0.5)
+    ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run plain copies of one vector (for the "
                          "HBM counter calibration of rocprofv3 --pmc)")
@@ -230,6 +327,8 @@ def main():
     del op
     if not args.no_irk:
         out["irk"] = time_irk(args)
+    if args.serving:
+        out["serving"] = time_serving(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))

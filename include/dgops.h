@@ -300,6 +300,56 @@ int dg_export_cell_vertices(const dg_operator *op, double *vertices);
 int dg_host_index_map(const dg_config *cfg, int64_t *n_cells,
                       double *cell_centres, int64_t *dof_offsets);
 
+/* ---- serving: many independent explicit solves on one GPU -------------
+ * No counterpart in the reference (its drivers run one solve per process);
+ * this is the request loop of the serving workload: 3D periodic unit
+ * cubes with a constant transport speed and Gaussian initial data, LSRK(5,4)
+ * to final_time with the time step rule of advection_solver.cc:1163-1167.
+ * Requests are admitted first come, first served while their device
+ * memory fits the budget; every dg_server_step advances all admitted
+ * requests by one time step, with one kernel launch per LSRK stage for all
+ * requests of one degree (continuous batching: requests join and leave at
+ * step boundaries). A request's result is bit-identical to the same solve
+ * run alone with dg_lsrk_time_step.                                      */
+typedef struct dg_serve_request
+{
+  int    degree;              /* 1..7                                       */
+  int    cells_per_direction; /* c 2^r with c in {1, 3, 5, 7}               */
+  double velocity[3];         /* constant transport speed                  */
+  double center[3], width;    /* exp(-|x - center|^2 / width^2)           */
+  double final_time;
+  double courant;             /* dt = C h / |a| / k^1.5, then shortened    */
+} dg_serve_request;
+
+enum dg_request_state
+{
+  DG_REQUEST_QUEUED  = 0,
+  DG_REQUEST_ACTIVE  = 1,
+  DG_REQUEST_DONE    = 2,
+  DG_REQUEST_UNKNOWN = 3
+};
+
+typedef struct dg_server dg_server;
+
+/* the operator configuration, time step and step count of a request (host
+ * only; the same solve can be run alone through dg_create)              */
+int dg_serve_request_config(const dg_serve_request *r, dg_config *cfg,
+                            double *time_step, int *n_steps);
+int dg_server_create(int64_t hbm_budget_bytes, void *stream,
+                     dg_server **server);
+int dg_server_destroy(dg_server *s);
+int dg_server_submit(dg_server *s, const dg_serve_request *r, int64_t *id);
+/* admit, advance every active request by one step, retire finished ones */
+int dg_server_step(dg_server *s, int *n_active, int *n_queued);
+int dg_server_request_info(dg_server *s, int64_t id, int *state,
+                           int64_t *n_dofs, int *steps_done,
+                           int *steps_total);
+/* copy a finished request's solution (n_dofs doubles, device pointer dst)
+ * and release it                                                         */
+int dg_server_take_result(dg_server *s, int64_t id, void *dst);
+/* device memory held by admitted and finished requests                  */
+int dg_server_memory(dg_server *s, int64_t *bytes_in_use);
+
 #ifdef __cplusplus
 }
 #endif

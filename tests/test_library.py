@@ -77,3 +77,22 @@ def test_host_index_map_tool_matches_oracle():
     assert np.array_equal(rows[:, 0], np.arange(o.n_cells))
     assert np.max(np.abs(rows[:, 1:4] - v.mean(axis=1))) < 1e-15
     assert np.array_equal(rows[:, 4], np.arange(o.n_cells) * 125)
+
+
+def test_serve_request_config_without_gpu():
+    """Request -> operator configuration (host only): 24 cells per direction
+    are 3^3 coarse cells refined 3 times; the time step follows the explicit
+    rule and divides the end time."""
+    import numpy as np
+    from advection_miniapp_amd.serving import request_config, serve_request
+    r = serve_request(4, 24, [0.6, 0.0, 0.8], [0.5, 0.5, 0.5], 0.1,
+                      final_time=0.5, courant=0.3)
+    cfg, dt, n = request_config(r)
+    assert (cfg.dim, cfg.degree, cfg.refine) == (3, 4, 3)
+    assert list(cfg.n_coarse) == [3, 3, 3] and list(cfg.periodic) == [1] * 3
+    dt0 = 0.3 * (1 / 24) / 1.0 / 4 ** 1.5
+    assert n == int(np.ceil(0.5 / dt0)) and abs(n * dt - 0.5) < 1e-14
+    import pytest
+    bad = serve_request(4, 12 * 11, [1, 0, 0], [0.5] * 3, 0.1)
+    with pytest.raises(_lib.DgError):
+        request_config(bad)
