@@ -146,7 +146,8 @@ SIGNATURES = {
     "dg_server_destroy": (_i, [_vp]),
     "dg_server_submit": (_i, [_vp, ctypes.POINTER(DgServeRequest),
                               ctypes.POINTER(_i64)]),
-    "dg_server_step": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "dg_server_step": (_i, [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i),
+                            ctypes.POINTER(_i)]),
     "dg_server_request_info": (_i, [_vp, _i64, ctypes.POINTER(_i),
                                     ctypes.POINTER(_i64), ctypes.POINTER(_i),
                                     ctypes.POINTER(_i)]),

@@ -180,12 +180,20 @@ namespace dgops
         }
     }
 
-    void
-    step()
+    // advance all active solves by up to max_steps steps (stopping at the
+    // first completion); returns the number of solves that finished
+    int
+    step(int max_steps)
     {
       admit();
       if (active.empty())
-        return;
+        return 0;
+      int n_run = std::max(1, max_steps);
+      for (int64_t id : active)
+        {
+          const ServeReq &q = *reqs.at(id);
+          n_run = std::min(n_run, q.n_steps - q.step);
+        }
       // groups of one degree, in admission order inside a group
       std::map<int, std::vector<ServeReq *>> groups;
       for (int64_t id : active)
@@ -193,7 +201,7 @@ namespace dgops
           ServeReq &q = *reqs.at(id);
           groups[q.cfg.degree + 1].push_back(&q);
         }
-      const size_t n_items = plan.size() * active.size();
+      const size_t n_items = (size_t)n_run * plan.size() * active.size();
       if (n_items > h_cap)
         {
           if (h_items)
@@ -215,39 +223,40 @@ namespace dgops
       };
       std::vector<Launch> launches;
       size_t              k = 0;
-      for (const LsrkStage &s : plan)
-        for (auto &g : groups)
-          {
-            const int n   = g.first;
-            auto      it  = batch_registry().find(batch_key(3, n));
-            require(it != batch_registry().end(),
-                    "batched operator not built for degree " +
-                      std::to_string(n - 1),
-                    DG_ERR_UNSUPPORTED);
-            const int64_t cpb   = block_cells(n * n);
-            Launch        L     = {it->second, k, 0, 0,
+      for (int j = 0; j < n_run; ++j)
+        for (const LsrkStage &s : plan)
+          for (auto &g : groups)
+            {
+              const int n  = g.first;
+              auto      it = batch_registry().find(batch_key(3, n));
+              require(it != batch_registry().end(),
+                      "batched operator not built for degree " +
+                        std::to_string(n - 1),
+                      DG_ERR_UNSUPPORTED);
+              const int64_t cpb = block_cells(n * n);
+              Launch        L   = {it->second, k, 0, 0,
                                    g.second.front()->op()->d_basis_d.ptr};
-            unsigned      first = 0;
-            for (ServeReq *q : g.second)
-              {
-                const double      t  = q->step * q->dt;
-                BatchItem<double> &b = h_items[k++];
-                b                    = {};
-                b.P                  = q->op()->base_params(t + s.c * q->dt, 1);
-                b.io.src             = q->reg[s.r_in].as<double>();
-                b.io.sol_in          = q->reg[s.s_in].as<double>();
-                b.io.sol_out         = q->reg[s.s_out].as<double>();
-                b.io.ri_out = s.last ? nullptr : q->reg[s.r_out].as<double>();
-                b.io.b_dt   = s.b * q->dt;
-                b.io.a_dt   = s.a * q->dt;
-                b.io.last   = s.last;
-                b.first_block = first;
-                first += (unsigned)((q->n_cells + cpb - 1) / cpb);
-                ++L.count;
-              }
-            L.blocks = first;
-            launches.push_back(L);
-          }
+              unsigned      first = 0;
+              for (ServeReq *q : g.second)
+                {
+                  const double       t = (q->step + j) * q->dt;
+                  BatchItem<double> &b = h_items[k++];
+                  b                    = {};
+                  b.P           = q->op()->base_params(t + s.c * q->dt, 1);
+                  b.io.src      = q->reg[s.r_in].as<double>();
+                  b.io.sol_in   = q->reg[s.s_in].as<double>();
+                  b.io.sol_out  = q->reg[s.s_out].as<double>();
+                  b.io.ri_out   = s.last ? nullptr : q->reg[s.r_out].as<double>();
+                  b.io.b_dt     = s.b * q->dt;
+                  b.io.a_dt     = s.a * q->dt;
+                  b.io.last     = s.last;
+                  b.first_block = first;
+                  first += (unsigned)((q->n_cells + cpb - 1) / cpb);
+                  ++L.count;
+                }
+              L.blocks = first;
+              launches.push_back(L);
+            }
       hip_check(hipMemcpyAsync(d_items.ptr, h_items,
                                sizeof(BatchItem<double>) * k,
                                hipMemcpyHostToDevice, stream),
@@ -256,13 +265,16 @@ namespace dgops
       for (const Launch &L : launches)
         hip_check(L.f(dev + L.first, L.count, L.blocks, L.basis, stream),
                   "batched operator launch");
+      // the pinned descriptors are reused by the next call
       hip_check(hipStreamSynchronize(stream), "serving step");
       // retire finished solves: keep the solution register only
       std::vector<int64_t> still;
+      int                  finished = 0;
       for (int64_t id : active)
         {
           ServeReq &q = *reqs.at(id);
-          if (++q.step < q.n_steps)
+          q.step += n_run;
+          if (q.step < q.n_steps)
             {
               still.push_back(id);
               continue;
@@ -273,8 +285,10 @@ namespace dgops
           dg_destroy(q.handle);
           q.handle = nullptr;
           used -= 2 * q.vec_bytes + q.op_bytes;
+          ++finished;
         }
       active.swap(still);
+      return finished;
     }
   };
 } // namespace dgops
@@ -333,12 +347,15 @@ dg_server_submit(dg_server *hs, const dg_serve_request *r, int64_t *id)
 }
 
 int
-dg_server_step(dg_server *hs, int *n_active, int *n_queued)
+dg_server_step(dg_server *hs, int max_steps, int *n_active, int *n_queued,
+               int *n_finished)
 {
   return guard([&] {
     require(hs != nullptr, "null server");
-    Server &S = *reinterpret_cast<Server *>(hs);
-    S.step();
+    Server   &S = *reinterpret_cast<Server *>(hs);
+    const int f = S.step(max_steps);
+    if (n_finished)
+      *n_finished = f;
     if (n_active)
       *n_active = (int)S.active.size();
     if (n_queued)

@@ -61,13 +61,15 @@ class Server:
                                            ctypes.byref(i)))
         return i.value
 
-    def step(self):
-        """Admit, advance every active solve by one step; returns
-        (n_active, n_queued) after the step."""
-        a, q = ctypes.c_int(), ctypes.c_int()
-        L.check(self._lib.dg_server_step(self._h, ctypes.byref(a),
-                                         ctypes.byref(q)))
-        return a.value, q.value
+    def step(self, max_steps=1):
+        """Admit, advance every active solve by up to max_steps steps
+        (returning at the first completion); returns (n_active, n_queued,
+        n_finished) after the call."""
+        a, q, f = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        L.check(self._lib.dg_server_step(self._h, int(max_steps),
+                                         ctypes.byref(a), ctypes.byref(q),
+                                         ctypes.byref(f)))
+        return a.value, q.value, f.value
 
     def info(self, rid):
         st, nd = ctypes.c_int(), ctypes.c_int64()

@@ -218,12 +218,14 @@ def time_serving(args):
         if len(ids) == len(done):
             time.sleep(max(0.0, arrivals[nxt] - now))
             continue
-        srv.step()
+        _, _, fin = srv.step(max_steps=args.serve_steps_per_call)
         now = time.perf_counter() - t0
-        for rid, j in ids.items():
-            if j not in done and srv.info(rid)["state"] == L.DG_REQUEST_DONE:
-                done[j] = now
-                srv.take_result(rid)
+        if fin:
+            for rid, j in ids.items():
+                if j not in done and \
+                        srv.info(rid)["state"] == L.DG_REQUEST_DONE:
+                    done[j] = now
+                    srv.take_result(rid)
     lat_b = np.array([done[j] - arrivals[j] for j in range(len(reqs))])
     span_b = max(done.values()) - arrivals[0]
     del srv
@@ -276,6 +278,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-final-time", type=float, default=0.5)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
+    ap.add_argument("--serve-steps-per-call", type=int, default=16)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run plain copies of one vector (for the "
                          "HBM counter calibration of rocprofv3 --pmc)")

@@ -339,8 +339,12 @@ int dg_server_create(int64_t hbm_budget_bytes, void *stream,
                      dg_server **server);
 int dg_server_destroy(dg_server *s);
 int dg_server_submit(dg_server *s, const dg_serve_request *r, int64_t *id);
-/* admit, advance every active request by one step, retire finished ones */
-int dg_server_step(dg_server *s, int *n_active, int *n_queued);
+/* admit what fits, then advance every active request by up to max_steps
+ * time steps (fewer when a request finishes earlier: the call returns at
+ * the first completion) without synchronizing in between, and retire the
+ * finished requests; n_finished counts them                              */
+int dg_server_step(dg_server *s, int max_steps, int *n_active, int *n_queued,
+                   int *n_finished);
 int dg_server_request_info(dg_server *s, int64_t id, int *state,
                            int64_t *n_dofs, int *steps_done,
                            int *steps_total);

@@ -43,14 +43,17 @@ def solo(r):
 def run_server(budget, reqs):
     srv = Server(budget)
     ids = [srv.submit(r) for r in reqs]
-    max_active, steps = 0, 0
+    max_active, calls, finished = 0, 0, 0
     while True:
-        a, q = srv.step()
-        max_active = max(max_active, a)
-        steps += 1
+        # a few steps per call: the call returns early at a completion
+        a, q, f = srv.step(max_steps=7)
+        finished += f
+        max_active = max(max_active, a + f)
+        calls += 1
         if a == 0 and q == 0:
             break
-        assert steps < 10000
+        assert calls < 10000
+    assert finished == len(reqs)
     out = []
     for i in ids:
         assert srv.info(i)["state"] == L.DG_REQUEST_DONE
