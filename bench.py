@@ -263,6 +263,67 @@ def time_serving(args):
                     makespan_s=span_s, dof_per_s=5 * tot / span_s))
 
 
+def time_one_apply(cfg_kw, steps=20, src=None):
+    """Average apply time (HIP events on the launch stream) of one
+    operator configuration; returns (operator, src, dst, ms)."""
+    import torch
+    from advection_miniapp_amd import AdvectionOperation, make_config
+    op = AdvectionOperation(make_config(**cfg_kw))
+    if src is None:
+        src = op.initialize_dof_vector()
+        op.project_initial(src, 0.0)
+    dst = op.initialize_dof_vector()
+    for _ in range(3):
+        op.apply(src, dst, 0.0)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(steps):
+        op.apply(src, dst, 0.0)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    return op, src, dst, start.elapsed_time(stop) / steps
+
+
+def time_extra():
+    """Configuration 3 (advection + SIPG diffusion, 32^3 cells, degrees
+    3/5/7, FP64 and FP32 operators, FP32 error against FP64) and the
+    Burgers add-on of configuration 4 (degree 3, 48^3 cells on [-1,1]^3,
+    u0 = (-sin pi x, 0, 0))."""
+    import math
+    import torch
+    out = {"sipg": {}, "burgers": {}}
+    for k in (3, 5, 7):
+        base = dict(dim=3, degree=k, refine=5, velocity=[1.0, 0.5, 0.25],
+                    operator_kind=1, viscosity=1e-3,
+                    penalty_factor=(k + 1) ** 2, solution_kind=2,
+                    solution_param=[2 * math.pi, 1e-3])
+        op, src, dst64, ms64 = time_one_apply(base)
+        n = op.n_dofs
+        del op
+        _, _, dst32, ms32 = time_one_apply(dict(base, precision=1),
+                                           src=src.float())
+        err = ((dst32.double() - dst64).abs().max() /
+               dst64.abs().max()).item()
+        out["sipg"][f"k{k}"] = dict(
+            n_dofs=n, fp64_ms=ms64, fp64_dof_per_s=n / (ms64 * 1e-3),
+            fp32_ms=ms32, fp32_dof_per_s=n / (ms32 * 1e-3),
+            fp32_rel_err_vs_fp64=err)
+        del src, dst64, dst32
+        torch.cuda.empty_cache()
+    bk = dict(dim=3, degree=3, n_components=3, operator_kind=2,
+              n_coarse=[3, 3, 3], refine=4, lo=[-1, -1, -1], hi=[1, 1, 1],
+              solution_kind=6)
+    op, _, _, ms = time_one_apply(bk)
+    out["burgers"] = dict(workload="3D Burgers, degree 3, 48^3 cells, "
+                                   "3 components, over-integration 6^3",
+                          unknowns=op.vector_size, ms=ms,
+                          unknowns_per_s=op.vector_size / (ms * 1e-3))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -273,6 +334,9 @@ def main():
     ap.add_argument("--irk-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-irk", action="store_true")
+    ap.add_argument("--extra", action="store_true",
+                    help="also time the SIPG (k = 3, 5, 7) and Burgers "
+                         "operators")
     ap.add_argument("--serving", action="store_true",
                     help="also run the serving workload (tens of seconds)")
     ap.add_argument("--serve-requests", type=int, default=48)
@@ -332,6 +396,8 @@ def main():
         out["irk"] = time_irk(args)
     if args.serving:
         out["serving"] = time_serving(args)
+    if args.extra:
+        out["extra"] = time_extra()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out))
