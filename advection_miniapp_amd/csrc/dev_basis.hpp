@@ -17,8 +17,9 @@ namespace dgops
     T l0[max_n], l1[max_n];     // Gauss-Lagrange values at 0 and 1
     T dl0[max_n], dl1[max_n];   // Gauss-Lagrange derivatives at 0 and 1
     T dphi0[max_n], dphi1[max_n];
-    T w[max_n];  // Gauss weights on [0,1]
-    T xq[max_n]; // Gauss points on [0,1]
+    T w[max_n];    // Gauss weights on [0,1]
+    T winv[max_n]; // their reciprocals (0 past n)
+    T xq[max_n];   // Gauss points on [0,1]
     T xn[max_n]; // Gauss-Lobatto nodes on [0,1]
   };
 
@@ -43,6 +44,7 @@ namespace dgops
         d.dphi0[i] = (T)b.dphi0[i];
         d.dphi1[i] = (T)b.dphi1[i];
         d.w[i]     = (T)b.gauss_wts[i];
+        d.winv[i]  = i < b.n ? (T)(1. / b.gauss_wts[i]) : (T)0;
         d.xq[i]    = (T)b.gauss_pts[i];
         d.xn[i]    = (T)b.gl_pts[i];
       }

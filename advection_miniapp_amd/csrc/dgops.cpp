@@ -274,13 +274,14 @@ namespace dgops
       throw Error(DG_ERR_UNSUPPORTED,
                   "advection-diffusion (SIPG) is built for axis-aligned "
                   "meshes (perturbation 0)");
-    // analytic geometry only where it is free (axis-aligned cells and a
-    // constant transport speed, or SIPG); everything else reads the
-    // precomputed geometry tables
-    const bool analytic =
-      diff || (P.cartesian && op.cfg.velocity_kind == DG_VEL_CONSTANT);
-    OpParams Pk = P;
-    if (!analytic)
+    // axis-aligned cells with a constant transport speed need no tables
+    // (products of 1D weights); SIPG with a variable speed evaluates it per
+    // point; everything else reads the precomputed geometry tables
+    const bool uniform =
+      P.cartesian && op.cfg.velocity_kind == DG_VEL_CONSTANT;
+    const int geo = uniform ? geo_uniform : diff ? geo_analytic : geo_table;
+    OpParams  Pk  = P;
+    if (geo == geo_table)
       {
         Operator &m = const_cast<Operator &>(op);
         aux_geometry_tables(m, fp32 ? DG_FP32 : DG_FP64);
@@ -292,8 +293,7 @@ namespace dgops
         aux_boundary_values(m, Pk);
       }
     const uint32_t key = op_key(op.cfg.dim, op.n, nc, fp32 ? 1 : 0,
-                                analytic ? 1 : 0, skew ? 1 : 0, diff, fin,
-                                epi);
+                                geo, skew ? 1 : 0, diff, fin, epi);
     auto it = op_registry().find(key);
     if (it == op_registry().end())
       throw Error(DG_ERR_UNSUPPORTED,
@@ -302,7 +302,9 @@ namespace dgops
                     " degree=" + std::to_string(op.n - 1) +
                     " components=" + std::to_string(nc) +
                     (fp32 ? " fp32" : " fp64") +
-                    (analytic ? " analytic geometry" : " tabulated geometry") +
+                    (geo == geo_uniform    ? " uniform geometry"
+                     : geo == geo_analytic ? " analytic geometry"
+                                           : " tabulated geometry") +
                     (skew ? " skew" : "") + " fin=" + std::to_string(fin) +
                     " epi=" + std::to_string(epi));
     hip_check(it->second(Pk, op.basis_ptr(fp32), op.d_basis_d.ptr, io,

@@ -9,37 +9,42 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 OUT = os.path.join(HERE, "generated")
 
-# operator variant: (dim, n, nc, fp32, cart, skew, diff, fin, epi); cart 1
-# evaluates the geometry analytically (axis-aligned cells with a constant
-# speed, or SIPG), cart 0 reads the precomputed geometry tables
+# operator variant: (dim, n, nc, fp32, geo, skew, diff, fin, epi); geo 0
+# reads the precomputed geometry tables, geo 1 evaluates axis-aligned
+# geometry and the speed per point (SIPG with a variable speed), geo 2 is
+# axis-aligned with a constant speed (products of 1D weights)
 FIN_INVMASS, FIN_INTEGRATE = 0, 1
 EPI_STORE, EPI_LSRK = 0, 1
+GEO_TABLE, GEO_ANALYTIC, GEO_UNIFORM = 0, 1, 2
+ADVECTION_GEO = (GEO_TABLE, GEO_UNIFORM)
+SIPG_GEO = (GEO_ANALYTIC, GEO_UNIFORM)
 
 
 def op_variants():
     v = []
     # explicit right-hand side M^-1(-L u): apply and fused LSRK stage
     for dim, n, cart, epi in itertools.product(
-            (2, 3), range(2, 9), (0, 1), (EPI_STORE, EPI_LSRK)):
+            (2, 3), range(2, 9), ADVECTION_GEO, (EPI_STORE, EPI_LSRK)):
         v.append((dim, n, 1, 0, cart, 0, 0, FIN_INVMASS, epi))
     # fp32 explicit right-hand side
-    for dim, n, cart in itertools.product((2, 3), range(2, 9), (0, 1)):
+    for dim, n, cart in itertools.product((2, 3), range(2, 9), ADVECTION_GEO):
         v.append((dim, n, 1, 1, cart, 0, 0, FIN_INVMASS, EPI_STORE))
     # implicit single-stage operator (M/dt + L) and IRK right-hand side,
     # skew-symmetric form, fp64 and fp32
     for dim, n, cart, fp32 in itertools.product(
-            (2, 3), range(2, 9), (0, 1), (0, 1)):
+            (2, 3), range(2, 9), ADVECTION_GEO, (0, 1)):
         v.append((dim, n, 1, fp32, cart, 1, 0, FIN_INTEGRATE, EPI_STORE))
     # stage-coupled IRK operator
     for dim, n, nc, fp32, cart in itertools.product(
-            (2, 3), range(2, 8), (2, 3, 4), (0, 1), (0, 1)):
+            (2, 3), range(2, 8), (2, 3, 4), (0, 1), ADVECTION_GEO):
         v.append((dim, n, nc, fp32, cart, 1, 0, FIN_INTEGRATE, EPI_STORE))
     # advection + SIPG diffusion, axis-aligned: explicit (apply, LSRK stage)
     # and implicit operator, fp64 and fp32
-    for dim, n, fp32 in itertools.product((2, 3), range(2, 9), (0, 1)):
-        v.append((dim, n, 1, fp32, 1, 0, 1, FIN_INVMASS, EPI_STORE))
-        v.append((dim, n, 1, fp32, 1, 0, 1, FIN_INVMASS, EPI_LSRK))
-        v.append((dim, n, 1, fp32, 1, 1, 1, FIN_INTEGRATE, EPI_STORE))
+    for dim, n, fp32, geo in itertools.product((2, 3), range(2, 9), (0, 1),
+                                               SIPG_GEO):
+        v.append((dim, n, 1, fp32, geo, 0, 1, FIN_INVMASS, EPI_STORE))
+        v.append((dim, n, 1, fp32, geo, 0, 1, FIN_INVMASS, EPI_LSRK))
+        v.append((dim, n, 1, fp32, geo, 1, 1, FIN_INTEGRATE, EPI_STORE))
     return v
 
 
@@ -94,12 +99,12 @@ def emit(path, variants):
                 f"  static FdmRegistrar f{i}(fdm_key({dim}, {n}, {ns}, {prec}), "
                 f"&launch_fdm<{dim}, {n}, {ns}, {tc}, {ti}, {to}>);")
             continue
-        dim, n, nc, fp32, cart, skew, diff, fin, epi = var
+        dim, n, nc, fp32, geo, skew, diff, fin, epi = var
         T = "float" if fp32 else "double"
         lines.append(
-            f"  static OpRegistrar r{i}(op_key({dim}, {n}, {nc}, {fp32}, {cart}, "
+            f"  static OpRegistrar r{i}(op_key({dim}, {n}, {nc}, {fp32}, {geo}, "
             f"{skew}, {diff}, {fin}, {epi}), &launch_op<{dim}, {n}, {nc}, {T}, "
-            f"{b(cart)}, {b(skew)}, {b(diff)}, {fin}, {epi}>);")
+            f"{geo}, {b(skew)}, {b(diff)}, {fin}, {epi}>);")
     lines += ["} // namespace dgops", ""]
     text = "\n".join(lines)
     if os.path.exists(path) and open(path).read() == text:

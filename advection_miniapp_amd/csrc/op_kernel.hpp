@@ -121,6 +121,15 @@ namespace dgops
     return x < rem ? x * (full + 1) + i : rem * (full + 1) + (x - rem) * full + i;
   }
 
+  // how a kernel obtains the geometry and transport speed at its points
+  enum Geometry : int
+  {
+    geo_table    = 0, // precomputed w adj(J) a, 1/JxW and face tables
+    geo_analytic = 1, // axis-aligned cells, speed evaluated per point
+    geo_uniform  = 2  // axis-aligned cells, constant speed: products of
+                      // 1D weights and per-mesh constants
+  };
+
   // cells per workgroup: two wavefronts' worth of lines, so that every
   // barrier holds back two waves only and more workgroups share a CU
   // (3D degree 4: 5 cells, 125 of 128 lanes busy)
@@ -326,7 +335,7 @@ namespace dgops
   // advection_solver_implicit_real.cc:1032-1058:
   //   F = 1/2 [(u- + u+) an + alpha |an| (u- - u+)] - skew an u-
   // with an = a . n dA on this cell's outward normal.
-  template <int dim, int n, int dir, typename T, bool cartesian, bool diff>
+  template <int dim, int n, int dir, typename T, int geo, bool diff>
   __device__ __forceinline__ void
   face_terms(const OpParams &P,
              const DevBasis<double> *Bd,
@@ -358,11 +367,11 @@ namespace dgops
         T up = uplus[f];
         if (active && nb[f] < 0)
           {
-            if constexpr (cartesian)
+            if constexpr (geo != geo_table)
               {
                 double an, x[3] = {0, 0, 0};
-                face_point_data<dim, cartesian>(P, X, Bd->xq, Bd->w, dir, s,
-                                                q, an, x);
+                face_point_data<dim, true>(P, X, Bd->xq, Bd->w, dir, s, q,
+                                           an, x);
                 up = (T)analytic_solution<dim>(P.sol_kind, P.sol_param,
                                                P.sol_time, P.vel_const, x);
               }
@@ -475,7 +484,7 @@ namespace dgops
   // The operator on the cells of workgroup `block` (one batch of
   // cells_per_block cells); P and io live in kernel arguments or, for the
   // batched launch, in global memory (uniform per workgroup either way).
-  template <int dim, int n, int NC, typename T, bool cartesian, bool skew,
+  template <int dim, int n, int NC, typename T, int geo, bool skew,
             bool diff, int fin, int epi>
   __device__ __forceinline__ void
   operator_body(const OpParams                       &P,
@@ -484,7 +493,7 @@ namespace dgops
                 const OpBuffers<T>                   &io,
                 unsigned                              block)
   {
-    static_assert(!diff || cartesian, "SIPG needs axis-aligned cells");
+    static_assert(!diff || geo != geo_table, "SIPG needs axis-aligned cells");
     using L            = Lay<dim, n>;
     constexpr int nl   = L::nl;
     constexpr int nd   = L::nd;
@@ -493,12 +502,14 @@ namespace dgops
     constexpr int FSTR = (diff ? 2 : 1) * 2 * dim * nl;
     constexpr int NV   = 1 << dim;
     constexpr int LAST = dim - 1;
-    constexpr int NX   = cartesian ? dim : NV * dim;
+    constexpr int NX   = geo == geo_table ? NV * dim : dim;
 
     // analytic (axis-aligned) geometry evaluates the transport speed at
     // LAST-line points and hands the other directions over through LDS;
-    // tabulated geometry reads every direction in its own line layout
-    constexpr bool AN = cartesian;
+    // tabulated and uniform geometry produce every direction in its own
+    // line layout
+    constexpr bool AN  = geo == geo_analytic;
+    constexpr bool UNI = geo == geo_uniform;
     __shared__ T      s_u[CPB * STR];
     __shared__ T      s_acc[CPB * STR];
     __shared__ T      s_g0[AN ? CPB * STR : 1];
@@ -762,7 +773,51 @@ namespace dgops
     // handed over through LDS
     T cv[3][n], jxw[n];
     T fan[2 * dim];
-    if constexpr (!AN)
+    // uniform geometry: w adj(J) a at point (i, line p of direction d) is
+    // w_i wl_d(p) |K| / h_d a_d, with wl_d(p) the product of the 1D
+    // weights at the line's other coordinates; a . n dA on the faces of
+    // that line is -+ wl_d(p) |K| / h_d a_d
+    double vol = 0, cdir[3] = {0, 0, 0}, wl[3] = {1, 1, 1};
+    if constexpr (UNI)
+      {
+        vol = P.h[0] * P.h[1] * (dim == 3 ? P.h[2] : 1.);
+#pragma unroll
+        for (int d = 0; d < dim; ++d)
+          cdir[d] = vol / P.h[d] * P.vel_const[d];
+        int q[3];
+#pragma unroll
+        for (int d = 0; d < dim; ++d)
+          {
+            if (d == 0)
+              L::template idx<0>(p, 0, q);
+            else if (d == 1)
+              L::template idx<(dim > 1 ? 1 : 0)>(p, 0, q);
+            else
+              L::template idx<LAST>(p, 0, q);
+#pragma unroll
+            for (int e = 0; e < dim; ++e)
+              if (e != d)
+                wl[d] *= Bd->w[q[e]];
+          }
+      }
+    if constexpr (UNI)
+      {
+        const double cl = wl[LAST] * cdir[LAST];
+#pragma unroll
+        for (int i = 0; i < n; ++i)
+          {
+            cv[LAST][i] = (T)(Bd->w[i] * cl);
+            jxw[i] = fin == fin_integrate ? (T)(Bd->w[i] * wl[LAST] * vol)
+                                          : (T)1;
+          }
+#pragma unroll
+        for (int d = 0; d < dim; ++d)
+          {
+            fan[2 * d]     = (T)(-wl[d] * cdir[d]);
+            fan[2 * d + 1] = (T)(wl[d] * cdir[d]);
+          }
+      }
+    else if constexpr (!AN)
       {
         // tabulated: LAST-direction speed and JxW on this thread's line,
         // a . n dA on the faces it touches (lower face of this cell, upper
@@ -807,7 +862,7 @@ namespace dgops
       L::template idx<LAST>(p, 0, q);
       T fl[2] = {0, 0};
       if (active)
-        line_cell_data<dim, n, LAST, cartesian, T>(P, X, Bd, q, cv, jxw, fl);
+        line_cell_data<dim, n, LAST, true, T>(P, X, Bd, q, cv, jxw, fl);
       else
         {
 #pragma unroll
@@ -826,7 +881,7 @@ namespace dgops
             const int d = dim == 3 ? task / (2 * n) : 0;
             const int s = dim == 3 ? (task / n) % 2 : task;
             const int j = dim == 3 ? task % n : 0;
-            face_line_data<dim, n, LAST, cartesian, T>(
+            face_line_data<dim, n, LAST, true, T>(
               P, X, Bd, d, s, j, fan_l + (2 * d + s) * nl + j * n);
           }
       __syncthreads();
@@ -839,7 +894,16 @@ namespace dgops
     // speed rows of the y and x phases, read once for all components and
     // in flight while the LAST phase computes
     T cvy[n], cvx[n];
-    if constexpr (!AN)
+    if constexpr (UNI)
+      {
+#pragma unroll
+        for (int i = 0; i < n; ++i)
+          {
+            cvy[i] = (T)(Bd->w[i] * wl[dim == 3 ? 1 : 0] * cdir[1]);
+            cvx[i] = (T)(Bd->w[i] * wl[0] * cdir[0]);
+          }
+      }
+    else if constexpr (!AN)
       {
         if constexpr (dim == 3)
           table_line<dim, n, 1>(tab_c, cell, 1, p, active, cvy);
@@ -883,7 +947,7 @@ namespace dgops
             }
           if constexpr (diff)
             diffusion_cell_term<dim, n, LAST, T>(P, Bd, B, p, uqs[c], acc);
-          face_terms<dim, n, LAST, T, cartesian, diff>(P, Bd, B, X, cell, p,
+          face_terms<dim, n, LAST, T, geo, diff>(P, Bd, B, X, cell, p,
                                                        active,
                                                        nb, uplus[c], dplus[c],
                                                        uqs[c], acc, vf, fan,
@@ -953,7 +1017,7 @@ namespace dgops
               }
             if constexpr (diff)
               diffusion_cell_term<dim, n, 1, T>(P, Bd, B, p, uq, acc);
-            face_terms<dim, n, 1, T, cartesian, diff>(P, Bd, B, X, cell, p,
+            face_terms<dim, n, 1, T, geo, diff>(P, Bd, B, X, cell, p,
                                                       active,
                                                       nb, uplus[c], dplus[c],
                                                       uq, acc, vf, fan,
@@ -993,7 +1057,7 @@ namespace dgops
             }
           if constexpr (diff)
             diffusion_cell_term<dim, n, 0, T>(P, Bd, B, p, uq, acc);
-          face_terms<dim, n, 0, T, cartesian, diff>(P, Bd, B, X, cell, p,
+          face_terms<dim, n, 0, T, geo, diff>(P, Bd, B, X, cell, p,
                                                     active,
                                                     nb, uplus[c], dplus[c], uq,
                                                     acc, vf, fan, c == 0);
@@ -1011,6 +1075,13 @@ namespace dgops
                 lds_write_line<dim, n, LAST>(g0_l, p, ij);
                 __syncthreads();
                 lds_read_line<dim, n, 0>(g0_l, p, ij);
+              }
+            else if constexpr (UNI)
+              {
+                const T r = (T)(1. / (wl[0] * vol));
+#pragma unroll
+                for (int i = 0; i < n; ++i)
+                  ij[i] = B.winv[i] * r;
               }
             else
               table_line<dim, n, 0>(tab_c, cell, dim, p, active, ij);
@@ -1075,7 +1146,7 @@ namespace dgops
       }
   }
 
-  template <int dim, int n, int NC, typename T, bool cartesian, bool skew,
+  template <int dim, int n, int NC, typename T, int geo, bool skew,
             bool diff, int fin, int epi>
   __global__ void __launch_bounds__(256)
   dg_operator_kernel(const OpParams                       P,
@@ -1083,7 +1154,7 @@ namespace dgops
                      const DevBasis<double> *__restrict__ Bd,
                      const OpBuffers<T>                   io)
   {
-    operator_body<dim, n, NC, T, cartesian, skew, diff, fin, epi>(
+    operator_body<dim, n, NC, T, geo, skew, diff, fin, epi>(
       P, Bp, Bd, io, xcd_block(blockIdx.x, gridDim.x));
   }
 
@@ -1101,7 +1172,7 @@ namespace dgops
   // Several operators of one variant in a single launch (the serving loop
   // advances all active solves of one degree by one LSRK stage with it).
   // Items are sorted by first_block; the lookup is uniform per workgroup.
-  template <int dim, int n, int NC, typename T, bool cartesian, bool skew,
+  template <int dim, int n, int NC, typename T, int geo, bool skew,
             bool diff, int fin, int epi>
   __global__ void __launch_bounds__(256)
   dg_operator_batch_kernel(const BatchItem<T> *__restrict__      items,
@@ -1115,7 +1186,7 @@ namespace dgops
       if (items[k].first_block <= b)
         r = k;
     const BatchItem<T> &it = items[r];
-    operator_body<dim, n, NC, T, cartesian, skew, diff, fin, epi>(
+    operator_body<dim, n, NC, T, geo, skew, diff, fin, epi>(
       it.P, Bp, Bd, it.io, b - it.first_block);
   }
 } // namespace dgops

@@ -21,13 +21,13 @@ namespace dgops
                                   hipStream_t);
 
   inline uint32_t
-  op_key(int dim, int n, int nc, int fp32, int cart, int skew, int diff,
+  op_key(int dim, int n, int nc, int fp32, int geo, int skew, int diff,
          int fin, int epi)
   {
     return (uint32_t)dim | (uint32_t)n << 2 | (uint32_t)nc << 6 |
-           (uint32_t)fp32 << 10 | (uint32_t)cart << 11 |
-           (uint32_t)skew << 12 | (uint32_t)fin << 13 | (uint32_t)epi << 15 |
-           (uint32_t)diff << 17;
+           (uint32_t)fp32 << 10 | (uint32_t)geo << 11 |
+           (uint32_t)skew << 13 | (uint32_t)fin << 14 | (uint32_t)epi << 16 |
+           (uint32_t)diff << 18;
   }
 
   std::map<uint32_t, OpLaunch> &op_registry();
@@ -158,8 +158,8 @@ namespace dgops
     constexpr int nl  = Lay<dim, n>::nl;
     if (blocks == 0 || n_items <= 0)
       return hipSuccess;
-    hipLaunchKernelGGL((dg_operator_batch_kernel<dim, n, 1, double, true,
-                                                 false, false,
+    hipLaunchKernelGGL((dg_operator_batch_kernel<dim, n, 1, double,
+                                                 geo_uniform, false, false,
                                                  fin_inverse_mass, epi_lsrk>),
                        dim3(blocks), dim3(CPB * nl), 0, stream,
                        (const BatchItem<double> *)items, n_items,
@@ -168,7 +168,7 @@ namespace dgops
     return hipGetLastError();
   }
 
-  template <int dim, int n, int NC, typename T, bool cart, bool skew,
+  template <int dim, int n, int NC, typename T, int geo, bool skew,
             bool diff, int fin, int epi>
   hipError_t
   launch_op(const OpParams &P,
@@ -183,7 +183,7 @@ namespace dgops
     if (blocks <= 0)
       return hipSuccess;
     hipLaunchKernelGGL(
-      (dg_operator_kernel<dim, n, NC, T, cart, skew, diff, fin, epi>),
+      (dg_operator_kernel<dim, n, NC, T, geo, skew, diff, fin, epi>),
       dim3((unsigned)blocks),
       dim3(CPB * nl),
       0,

@@ -38,6 +38,22 @@ def test_sipg_apply_matches_oracle(gpu_lib, dim, k):
     assert rel_diff(host(dst), o.apply(u, 0.1)) < 1e-12
 
 
+@pytest.mark.parametrize("dim,k,vk", [(3, 4, 1), (2, 3, 2)])
+def test_sipg_variable_speed_matches_oracle(gpu_lib, dim, k, vk):
+    """A space-dependent speed takes the per-point speed evaluation (the
+    constant-speed cases above use products of 1D weights)."""
+    c = sipg(dim, k, velocity_kind=vk)
+    op, o = AdvectionOperation(c.dg()), c.oracle_handle()
+    u = np.random.default_rng(13).standard_normal(o.n_dofs)
+    dst = op.initialize_dof_vector()
+    op.apply(dev(u), dst, 0.3)
+    assert rel_diff(host(dst), o.apply(u, 0.3)) < 1e-12
+    op.set_time(0.2, 0.05)
+    dst = op.initialize_dof_vector()
+    op.vmult(dst, dev(u))
+    assert rel_diff(host(dst), o.vmult(u, 0.2, 0.05)) < 1e-12
+
+
 @pytest.mark.parametrize("k", [3, 5])
 def test_sipg_fp32_error(gpu_lib, k):
     c = sipg(3, k)
