@@ -21,10 +21,13 @@ namespace dgops
   // (CellwiseInverseMassMatrix::apply, advection_solver_implicit_real.cc:894-909)
   void aux_inverse_mass(const Operator &op, int p, void *vec);
 
-  // geometry tables of the operator kernels in precision p (see OpParams)
+  // boundary face index and face point tables (every geometry mode)
+  void aux_boundary_tables(Operator &op);
+  // geometry tables of the operator kernels in precision p (see OpParams);
+  // includes the boundary tables
   void aux_geometry_tables(Operator &op, int p);
   // analytic Dirichlet data (P.sol_kind at P.sol_time) on every boundary
-  // face point, for the tabulated-geometry kernels (no-op without boundary
+  // face point, read by the operator kernels (no-op without boundary
   // faces)
   void aux_boundary_values(Operator &op, const OpParams &P);
 

@@ -820,8 +820,7 @@ namespace dgops
 
     template <int dim, int n>
     void
-    launch_geometry_tables(Operator &op, const OpParams &P, const int *bcell,
-                           const int *bface, int64_t n_bnd)
+    launch_geometry_tables(Operator &op, const OpParams &P)
     {
       const auto *B = op.d_basis_d.as<DevBasis<double>>();
       if (op.mesh.n_cells > 0)
@@ -830,6 +829,14 @@ namespace dgops
                            op.stream, P, B, op.d_tab_cell[0].as<double>(),
                            op.d_tab_face[0].as<double>());
       hip_check(hipGetLastError(), "geometry tables");
+    }
+
+    template <int dim, int n>
+    void
+    launch_boundary_tables(Operator &op, const OpParams &P, const int *bcell,
+                           const int *bface, int64_t n_bnd)
+    {
+      const auto *B = op.d_basis_d.as<DevBasis<double>>();
       if (n_bnd > 0)
         hipLaunchKernelGGL((boundary_table_kernel<dim, n>), dim3(n_bnd),
                            dim3(64), 0, op.stream, P, B, bcell, bface,
@@ -896,64 +903,85 @@ namespace dgops
   }
 
   void
+  aux_boundary_tables(Operator &op)
+  {
+    if (op.boundary_ready)
+      return;
+    const int     dim = op.cfg.dim, n = op.n;
+    const int     nl  = dim == 3 ? n * n : n;
+    const int64_t nc  = op.mesh.n_cells;
+    // boundary faces in cell order
+    std::vector<int> bidx(std::max<int64_t>(1, nc * 2 * dim), -1);
+    std::vector<int> bcell, bface;
+    for (int64_t c = 0; c < nc; ++c)
+      for (int f = 0; f < 2 * dim; ++f)
+        if (op.mesh.neighbors[c * 2 * dim + f] < 0)
+          {
+            bidx[c * 2 * dim + f] = (int)bcell.size();
+            bcell.push_back((int)c);
+            bface.push_back(f);
+          }
+    const int64_t nb = (int64_t)bcell.size();
+    op.d_bnd_index.alloc(sizeof(int) * bidx.size());
+    hip_check(hipMemcpy(op.d_bnd_index.ptr, bidx.data(),
+                        sizeof(int) * bidx.size(), hipMemcpyHostToDevice),
+              "upload boundary index");
+    op.d_tab_bnd.alloc(sizeof(double) * (1 + dim) * nl *
+                       std::max<int64_t>(1, nb));
+    op.d_bnd_value.alloc(sizeof(double) * nl * std::max<int64_t>(1, nb));
+    op.n_bnd = nb;
+    if (nb > 0)
+      {
+        DevBuf d_bcell, d_bface;
+        d_bcell.alloc(sizeof(int) * nb);
+        d_bface.alloc(sizeof(int) * nb);
+        hip_check(hipMemcpy(d_bcell.ptr, bcell.data(), sizeof(int) * nb,
+                            hipMemcpyHostToDevice),
+                  "upload boundary cells");
+        hip_check(hipMemcpy(d_bface.ptr, bface.data(), sizeof(int) * nb,
+                            hipMemcpyHostToDevice),
+                  "upload boundary faces");
+        const OpParams P = op.base_params(0, 1);
+#define DG_BT(D, N)                                                        \
+  else if (dim == D && n == N)                                             \
+    launch_boundary_tables<D, N>(op, P, d_bcell.as<int>(), d_bface.as<int>(), \
+                                 nb);
+        if (false)
+          ;
+        DG_BT(2, 2) DG_BT(2, 3) DG_BT(2, 4) DG_BT(2, 5) DG_BT(2, 6)
+        DG_BT(2, 7) DG_BT(2, 8) DG_BT(3, 2) DG_BT(3, 3) DG_BT(3, 4)
+        DG_BT(3, 5) DG_BT(3, 6) DG_BT(3, 7) DG_BT(3, 8)
+#undef DG_BT
+        // the temporaries above are freed at scope end: finish first
+        hip_check(hipStreamSynchronize(op.stream), "boundary tables");
+      }
+    op.boundary_ready = true;
+  }
+
+  void
   aux_geometry_tables(Operator &op, int p)
   {
     const int pi = p == DG_FP32 ? 1 : 0;
     if (op.tables_ready[pi])
       return;
+    aux_boundary_tables(op);
     const int     dim = op.cfg.dim, n = op.n;
     const int     nl  = dim == 3 ? n * n : n;
     const int64_t nc  = op.mesh.n_cells;
     if (!op.tables_ready[0])
       {
-        // boundary faces in cell order
-        std::vector<int> bidx(std::max<int64_t>(1, nc * 2 * dim), -1);
-        std::vector<int> bcell, bface;
-        for (int64_t c = 0; c < nc; ++c)
-          for (int f = 0; f < 2 * dim; ++f)
-            if (op.mesh.neighbors[c * 2 * dim + f] < 0)
-              {
-                bidx[c * 2 * dim + f] = (int)bcell.size();
-                bcell.push_back((int)c);
-                bface.push_back(f);
-              }
-        const int64_t nb = (int64_t)bcell.size();
-        op.d_bnd_index.alloc(sizeof(int) * bidx.size());
-        hip_check(hipMemcpy(op.d_bnd_index.ptr, bidx.data(),
-                            sizeof(int) * bidx.size(), hipMemcpyHostToDevice),
-                  "upload boundary index");
-        DevBuf d_bcell, d_bface;
-        d_bcell.alloc(sizeof(int) * std::max<int64_t>(1, nb));
-        d_bface.alloc(sizeof(int) * std::max<int64_t>(1, nb));
-        if (nb > 0)
-          {
-            hip_check(hipMemcpy(d_bcell.ptr, bcell.data(), sizeof(int) * nb,
-                                hipMemcpyHostToDevice),
-                      "upload boundary cells");
-            hip_check(hipMemcpy(d_bface.ptr, bface.data(), sizeof(int) * nb,
-                                hipMemcpyHostToDevice),
-                      "upload boundary faces");
-          }
         op.d_tab_cell[0].alloc(sizeof(double) * (dim + 1) * op.n_dofs);
         op.d_tab_face[0].alloc(sizeof(double) * dim * nl *
                                std::max<int64_t>(1, nc));
-        op.d_tab_bnd.alloc(sizeof(double) * (1 + dim) * nl *
-                           std::max<int64_t>(1, nb));
-        op.d_bnd_value.alloc(sizeof(double) * nl * std::max<int64_t>(1, nb));
-        op.n_bnd = nb;
         const OpParams P = op.base_params(0, 1);
 #define DG_GT(D, N)                                                         \
-  else if (dim == D && n == N)                                              \
-    launch_geometry_tables<D, N>(op, P, d_bcell.as<int>(), d_bface.as<int>(), \
-                                 nb);
+  else if (dim == D && n == N) launch_geometry_tables<D, N>(op, P);
         if (false)
           ;
         DG_GT(2, 2) DG_GT(2, 3) DG_GT(2, 4) DG_GT(2, 5) DG_GT(2, 6)
         DG_GT(2, 7) DG_GT(2, 8) DG_GT(3, 2) DG_GT(3, 3) DG_GT(3, 4)
         DG_GT(3, 5) DG_GT(3, 6) DG_GT(3, 7) DG_GT(3, 8)
 #undef DG_GT
-        // the temporaries above are freed at scope end: finish first
-        hip_check(hipStreamSynchronize(op.stream), "geometry tables");
         op.tables_ready[0] = true;
       }
     if (pi == 1)

@@ -281,17 +281,19 @@ namespace dgops
       P.cartesian && op.cfg.velocity_kind == DG_VEL_CONSTANT;
     const int geo = uniform ? geo_uniform : diff ? geo_analytic : geo_table;
     OpParams  Pk  = P;
+    Operator &m   = const_cast<Operator &>(op);
     if (geo == geo_table)
       {
-        Operator &m = const_cast<Operator &>(op);
         aux_geometry_tables(m, fp32 ? DG_FP32 : DG_FP64);
-        Pk.tab_cell  = m.d_tab_cell[fp32 ? 1 : 0].ptr;
-        Pk.tab_face  = m.d_tab_face[fp32 ? 1 : 0].ptr;
-        Pk.tab_bnd   = m.d_tab_bnd.as<double>();
-        Pk.bnd_index = m.d_bnd_index.as<int>();
-        Pk.bnd_value = m.d_bnd_value.as<double>();
-        aux_boundary_values(m, Pk);
+        Pk.tab_cell = m.d_tab_cell[fp32 ? 1 : 0].ptr;
+        Pk.tab_face = m.d_tab_face[fp32 ? 1 : 0].ptr;
       }
+    else
+      aux_boundary_tables(m);
+    Pk.tab_bnd   = m.d_tab_bnd.as<double>();
+    Pk.bnd_index = m.d_bnd_index.as<int>();
+    Pk.bnd_value = m.d_bnd_value.as<double>();
+    aux_boundary_values(m, Pk);
     const uint32_t key = op_key(op.cfg.dim, op.n, nc, fp32 ? 1 : 0,
                                 geo, skew ? 1 : 0, diff, fin, epi);
     auto it = op_registry().find(key);

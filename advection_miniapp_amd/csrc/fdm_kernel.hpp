@@ -162,7 +162,7 @@ namespace dgops
   }
 
   template <int dim, int n, int NS, typename Tc, typename Tin, typename Tout>
-  __global__ void __launch_bounds__(256)
+  __global__ void __launch_bounds__(128)
   fdm_kernel(const FdmParams P, const FdmDev<Tc> *__restrict__ Fp,
              const Tin *__restrict__ src, Tout *__restrict__ dst)
   {

@@ -111,11 +111,14 @@ namespace dgops
     // tabulated geometry of the operator kernels ([0] fp64, [1] fp32):
     // built on first use, time independent (the velocity's time factor
     // is applied in the kernel)
-    DevBuf d_tab_cell[2], d_tab_face[2], d_tab_bnd, d_bnd_index;
+    DevBuf d_tab_cell[2], d_tab_face[2];
     bool   tables_ready[2] = {false, false};
-    // Dirichlet data at the boundary face points, refreshed per launch
-    DevBuf  d_bnd_value;
-    int64_t n_bnd = 0; // boundary faces
+    // boundary faces of every geometry mode: index per cell face, a . n dA
+    // and x per face point (built once), Dirichlet data at those points
+    // (refreshed per launch)
+    DevBuf  d_tab_bnd, d_bnd_index, d_bnd_value;
+    bool    boundary_ready = false;
+    int64_t n_bnd          = 0; // boundary faces
 
     // scratch for reductions and solvers
     DevBuf d_partials, d_result;

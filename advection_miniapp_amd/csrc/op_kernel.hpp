@@ -366,21 +366,11 @@ namespace dgops
           um = fma(l[i], uq[i], um);
         T up = uplus[f];
         if (active && nb[f] < 0)
-          {
-            if constexpr (geo != geo_table)
-              {
-                double an, x[3] = {0, 0, 0};
-                face_point_data<dim, true>(P, X, Bd->xq, Bd->w, dir, s, q,
-                                           an, x);
-                up = (T)analytic_solution<dim>(P.sol_kind, P.sol_param,
-                                               P.sol_time, P.vel_const, x);
-              }
-            else
-              // evaluated before the launch (boundary_value_kernel)
-              up = (T)P.bnd_value[(int64_t)P.bnd_index[cell * 2 * dim + f] *
-                                    Lay<dim, n>::nl +
-                                  p];
-          }
+          // Dirichlet data, evaluated before the launch
+          // (boundary_value_kernel)
+          up = (T)P.bnd_value[(int64_t)P.bnd_index[cell * 2 * dim + f] *
+                                Lay<dim, n>::nl +
+                              p];
         const T an = fan[f] * vf;
         const T flux =
           (T)0.5 * ((um + up) * an + (T)P.alpha * fabs(an) * (um - up)) -
@@ -1148,7 +1138,7 @@ namespace dgops
 
   template <int dim, int n, int NC, typename T, int geo, bool skew,
             bool diff, int fin, int epi>
-  __global__ void __launch_bounds__(256)
+  __global__ void __launch_bounds__(128)
   dg_operator_kernel(const OpParams                       P,
                      const DevBasis<T> *__restrict__      Bp,
                      const DevBasis<double> *__restrict__ Bd,
@@ -1174,7 +1164,7 @@ namespace dgops
   // Items are sorted by first_block; the lookup is uniform per workgroup.
   template <int dim, int n, int NC, typename T, int geo, bool skew,
             bool diff, int fin, int epi>
-  __global__ void __launch_bounds__(256)
+  __global__ void __launch_bounds__(128)
   dg_operator_batch_kernel(const BatchItem<T> *__restrict__      items,
                            int                                  n_items,
                            const DevBasis<T> *__restrict__      Bp,

@@ -69,6 +69,7 @@ namespace dgops
   {
     constexpr int CPB    = cells_per_block<dim, n>();
     constexpr int nl     = Lay<dim, n>::nl;
+    static_assert(CPB * nl <= 128, "kernels are bounded to 128 threads");
     const int64_t blocks = (P.n_cells + CPB - 1) / CPB;
     if (blocks <= 0)
       return hipSuccess;
@@ -156,6 +157,7 @@ namespace dgops
   {
     constexpr int CPB = cells_per_block<dim, n>();
     constexpr int nl  = Lay<dim, n>::nl;
+    static_assert(CPB * nl <= 128, "kernels are bounded to 128 threads");
     if (blocks == 0 || n_items <= 0)
       return hipSuccess;
     hipLaunchKernelGGL((dg_operator_batch_kernel<dim, n, 1, double,
@@ -179,6 +181,7 @@ namespace dgops
   {
     constexpr int CPB    = cells_per_block<dim, n>();
     constexpr int nl     = Lay<dim, n>::nl;
+    static_assert(CPB * nl <= 128, "kernels are bounded to 128 threads");
     const int64_t blocks = (P.n_cells + CPB - 1) / CPB;
     if (blocks <= 0)
       return hipSuccess;
