@@ -530,14 +530,30 @@ namespace dgops
     T      *fan_l = AN ? s_fan + lcs * 2 * (dim - 1) * nl : s_fan;
     double *X     = s_X + lcs * NX;
 
-    if (active)
-      for (int e = p; e < NX; e += nl)
-        X[e] = P.cell_vertices[cell * NV * dim + e];
+    // the cell's own values of the first component are requested first:
+    // the x sweep then overlaps the neighbour-index -> trace load chain
+    auto load_own = [&](const T *src, T(&r)[n]) {
+#pragma unroll
+      for (int i = 0; i < n; ++i)
+        r[i] = active ? src[cell * nd + L::template off<0>(p, i)] : (T)0;
+    };
+    T r0[n];
+    load_own(io.src, r0);
+    // cell vertices: analytic geometry, and JxW of the mass term with
+    // tabulated geometry
+    constexpr bool needX = AN || (geo == geo_table && fin == fin_integrate);
+    if constexpr (needX)
+      {
+        if (active)
+          for (int e = p; e < NX; e += nl)
+            X[e] = P.cell_vertices[cell * NV * dim + e];
+      }
     int nb[2 * dim];
 #pragma unroll
     for (int f = 0; f < 2 * dim; ++f)
       nb[f] = active ? P.neighbors[cell * 2 * dim + f] : -1;
-    __syncthreads();
+    if constexpr (needX)
+      __syncthreads();
 
     // ---- per component: neighbour traces and values at quadrature points
     T uplus[NC][2 * dim];
@@ -547,28 +563,46 @@ namespace dgops
     for (int c = 0; c < NC; ++c)
       {
         const T *src = io.src + c * io.comp_stride;
+        T        r[n], t[n];
+        if (c == 0)
+#pragma unroll
+          for (int i = 0; i < n; ++i)
+            r[i] = r0[i];
+        else
+          load_own(src, r);
+        // neighbours' face nodes for LDS staging (thread (i, j) loads node
+        // (t1 = i, t2 = j) of every face, one load per face); stored after
+        // the x sweep so that the sweep does not wait for them
+        T tv[6];
         if constexpr (dim == 3 && !diff)
           {
-            // stage the neighbours' face nodes in LDS: thread (i, j) loads
-            // node (t1 = i, t2 = j) of every face, one load per face
             const int i0 = p % n, j0 = p / n;
 #pragma unroll
             for (int f = 0; f < 6; ++f)
               {
                 const int d = f / 2, s = f % 2;
                 const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
-                T         v  = 0;
+                tv[f]        = 0;
                 if (active && nb[f] >= 0)
                   {
                     int ijk[3];
                     ijk[d]  = s ? 0 : n - 1;
                     ijk[t1] = i0;
                     ijk[t2] = j0;
-                    v       = src[(int64_t)nb[f] * nd + ijk[0] + n * ijk[1] +
-                            n * n * ijk[2]];
+                    tv[f]   = src[(int64_t)nb[f] * nd + ijk[0] + n * ijk[1] +
+                                n * n * ijk[2]];
                   }
-                f_l[f * nl + j0 * n + i0] = v;
               }
+          }
+        // own values, sweep along x
+        mat_apply<n>(B.S, r, t);
+        lds_write_line<dim, n, 0>(acc_l, p, t);
+        if constexpr (dim == 3 && !diff)
+          {
+            const int i0 = p % n, j0 = p / n;
+#pragma unroll
+            for (int f = 0; f < 6; ++f)
+              f_l[f * nl + j0 * n + i0] = tv[f];
           }
         else if constexpr (dim == 3)
           {
@@ -629,23 +663,6 @@ namespace dgops
                   }
               }
           }
-        // own values, sweep along x
-        T r[n], t[n];
-        if (active)
-          {
-            const T *csrc = src + cell * nd;
-#pragma unroll
-            for (int i = 0; i < n; ++i)
-              r[i] = csrc[L::template off<0>(p, i)];
-          }
-        else
-          {
-#pragma unroll
-            for (int i = 0; i < n; ++i)
-              r[i] = 0;
-          }
-        mat_apply<n>(B.S, r, t);
-        lds_write_line<dim, n, 0>(acc_l, p, t);
         __syncthreads();
         T fv[6];
         if constexpr (dim == 3 && !diff)
