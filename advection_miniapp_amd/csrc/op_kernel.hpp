@@ -555,6 +555,56 @@ namespace dgops
     if constexpr (needX)
       __syncthreads();
 
+    // tabulated geometry: LAST-direction speed on this thread's line, the
+    // y and x speed rows, and a . n dA on the faces its lines end on (lower
+    // face of this cell, upper face = lower face of the upper neighbour,
+    // boundary faces apart). Several components: requested here, ahead of
+    // the barriers of the value phase, so that their latency overlaps it
+    // (with one component the registers this holds cost more occupancy
+    // than the overlap gains)
+    T cv[3][n], jxw[n], cvy[n], cvx[n];
+    T fan[2 * dim];
+    auto load_tables = [&]() {
+        table_line<dim, n, LAST>(tab_c, cell, LAST, p, active, cv[LAST]);
+#pragma unroll
+        for (int d = 0; d < dim; ++d)
+          {
+            fan[2 * d] =
+              active ? -tab_f[(cell * dim + d) * nl + p] : (T)0;
+            T up = 0;
+            if (active)
+              {
+                const int nbu = nb[2 * d + 1];
+                if (nbu >= 0)
+                  up = tab_f[((int64_t)nbu * dim + d) * nl + p];
+                else
+                  up = (T)P.tab_bnd[((int64_t)P.bnd_index[cell * 2 * dim +
+                                                          2 * d + 1] *
+                                       nl +
+                                     p) *
+                                    (1 + dim)];
+              }
+            fan[2 * d + 1] = up;
+            if (active && nb[2 * d] < 0)
+              fan[2 * d] =
+                (T)P.tab_bnd[((int64_t)P.bnd_index[cell * 2 * dim + 2 * d] *
+                                nl +
+                              p) *
+                             (1 + dim)];
+          }
+    };
+    auto load_speed_rows = [&]() {
+      if constexpr (dim == 3)
+        table_line<dim, n, 1>(tab_c, cell, 1, p, active, cvy);
+      table_line<dim, n, 0>(tab_c, cell, 0, p, active, cvx);
+    };
+    constexpr bool early_tables = geo == geo_table && NC > 1;
+    if constexpr (early_tables)
+      {
+        load_tables();
+        load_speed_rows();
+      }
+
     // ---- per component: neighbour traces and values at quadrature points
     T uplus[NC][2 * dim];
     T dplus[NC][2 * dim];
@@ -778,8 +828,6 @@ namespace dgops
     // faces its line ends on; faces normal to the other directions are
     // evaluated along LAST-direction lines by up to one task per thread and
     // handed over through LDS
-    T cv[3][n], jxw[n];
-    T fan[2 * dim];
     // uniform geometry: w adj(J) a at point (i, line p of direction d) is
     // w_i wl_d(p) |K| / h_d a_d, with wl_d(p) the product of the 1D
     // weights at the line's other coordinates; a . n dA on the faces of
@@ -826,42 +874,15 @@ namespace dgops
       }
     else if constexpr (!AN)
       {
-        // tabulated: LAST-direction speed and JxW on this thread's line,
-        // a . n dA on the faces it touches (lower face of this cell, upper
-        // face = lower face of the upper neighbour, boundary faces apart)
-        table_line<dim, n, LAST>(tab_c, cell, LAST, p, active, cv[LAST]);
+        if constexpr (!early_tables)
+          load_tables();
+        // JxW of the mass term on this thread's line
         if (fin == fin_integrate && active && P.has_mass)
           line_jxw<dim, n, LAST, T, L>(X, Bd->xq, Bd->w, p, jxw);
         else
 #pragma unroll
           for (int i = 0; i < n; ++i)
             jxw[i] = 1;
-#pragma unroll
-        for (int d = 0; d < dim; ++d)
-          {
-            fan[2 * d] =
-              active ? -tab_f[(cell * dim + d) * nl + p] : (T)0;
-            T up = 0;
-            if (active)
-              {
-                const int nbu = nb[2 * d + 1];
-                if (nbu >= 0)
-                  up = tab_f[((int64_t)nbu * dim + d) * nl + p];
-                else
-                  up = (T)P.tab_bnd[((int64_t)P.bnd_index[cell * 2 * dim +
-                                                          2 * d + 1] *
-                                       nl +
-                                     p) *
-                                    (1 + dim)];
-              }
-            fan[2 * d + 1] = up;
-            if (active && nb[2 * d] < 0)
-              fan[2 * d] =
-                (T)P.tab_bnd[((int64_t)P.bnd_index[cell * 2 * dim + 2 * d] *
-                                nl +
-                              p) *
-                             (1 + dim)];
-          }
       }
     else
     {
@@ -898,9 +919,9 @@ namespace dgops
         fan[f] = fan_l[f * nl + fidx];
     }
 
-    // speed rows of the y and x phases, read once for all components and
-    // in flight while the LAST phase computes
-    T cvy[n], cvx[n];
+    // speed rows of the y and x phases, read once for all components
+    if constexpr (geo == geo_table && !early_tables)
+      load_speed_rows();
     if constexpr (UNI)
       {
 #pragma unroll
@@ -909,12 +930,6 @@ namespace dgops
             cvy[i] = (T)(Bd->w[i] * wl[dim == 3 ? 1 : 0] * cdir[1]);
             cvx[i] = (T)(Bd->w[i] * wl[0] * cdir[0]);
           }
-      }
-    else if constexpr (!AN)
-      {
-        if constexpr (dim == 3)
-          table_line<dim, n, 1>(tab_c, cell, 1, p, active, cvy);
-        table_line<dim, n, 0>(tab_c, cell, 0, p, active, cvx);
       }
 #pragma unroll
     for (int c = 0; c < NC; ++c)
