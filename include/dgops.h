@@ -306,8 +306,8 @@ int dg_host_index_map(const dg_config *cfg, int64_t *n_cells,
  * cubes with a constant transport speed and Gaussian initial data, LSRK(5,4)
  * to final_time with the time step rule of advection_solver.cc:1163-1167.
  * Requests are admitted first come, first served while their device
- * memory fits the budget; every dg_server_step advances all admitted
- * requests by one time step, with one kernel launch per LSRK stage for all
+ * memory fits the budget; dg_server_step advances all admitted requests
+ * time step by time step, with one kernel launch per LSRK stage for all
  * requests of one degree (continuous batching: requests join and leave at
  * step boundaries). A request's result is bit-identical to the same solve
  * run alone with dg_lsrk_time_step.                                      */

@@ -18,7 +18,8 @@ pytestmark = pytest.mark.gpu
 def requests():
     rng = np.random.default_rng(21)
     out = []
-    for k, N in ((2, 8), (3, 8), (3, 16), (4, 8), (5, 8), (6, 8), (4, 3)):
+    for k, N in ((2, 8), (3, 8), (3, 16), (4, 8), (5, 8), (6, 8), (4, 3),
+                 (1, 8), (7, 3)):
         a = rng.standard_normal(3)
         out.append(serve_request(k, N, a / np.linalg.norm(a),
                                  rng.uniform(0.2, 0.8, 3),
