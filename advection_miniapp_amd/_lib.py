@@ -113,6 +113,9 @@ SIGNATURES = {
     "dg_project_initial": (_i, [_vp, _vp, _d]),
     "dg_compute_mass_and_energy": (_i, [_vp, _vp, _dp]),
     "dg_l2_error": (_i, [_vp, _vp, _d, _dp]),
+    "dg_write_vtu": (_i, [_vp, _vp, _d, ctypes.c_char_p]),
+    "dg_host_write_vtu": (_i, [ctypes.POINTER(DgConfig), _dp, _d,
+                               ctypes.c_char_p]),
     "dg_set_time": (_i, [_vp, _d, _d]),
     "dg_vmult": (_i, [_vp, _vp, _vp]),
     "dg_apply_inverse_mass_matrix": (_i, [_vp, _vp]),

@@ -21,6 +21,10 @@ namespace dgops
   // (CellwiseInverseMassMatrix::apply, advection_solver_implicit_real.cc:894-909)
   void aux_inverse_mass(const Operator &op, int p, void *vec);
 
+  // VTU file of a host solution vector (vtu.cpp)
+  void write_vtu_host(const dg_config &cfg, const StructuredMesh &mesh,
+                      const double *values, double time, const char *filename);
+
   // boundary face index and face point tables (every geometry mode)
   void aux_boundary_tables(Operator &op);
   // geometry tables of the operator kernels in precision p (see OpParams);

@@ -812,6 +812,43 @@ dg_export_cell_vertices(const dg_operator *h, double *v)
 }
 
 int
+dg_write_vtu(dg_operator *h, const void *vec, double t, const char *filename)
+{
+  return guard([&] {
+    Operator *op = get(h);
+    require(vec && filename, "null argument");
+    const int64_t count = op->n_dofs * op->cfg.n_components;
+    std::vector<double> host(count);
+    hip_check(hipStreamSynchronize(op->stream), "sync");
+    if (op->fp32())
+      {
+        std::vector<float> f(count);
+        hip_check(hipMemcpy(f.data(), vec, sizeof(float) * count,
+                            hipMemcpyDeviceToHost),
+                  "copy solution");
+        for (int64_t i = 0; i < count; ++i)
+          host[i] = f[i];
+      }
+    else
+      hip_check(hipMemcpy(host.data(), vec, sizeof(double) * count,
+                          hipMemcpyDeviceToHost),
+                "copy solution");
+    write_vtu_host(op->cfg, op->mesh, host.data(), t, filename);
+  });
+}
+
+int
+dg_host_write_vtu(const dg_config *cfg, const double *vec, double t,
+                  const char *filename)
+{
+  return guard([&] {
+    require(cfg && vec && filename, "null argument");
+    validate(*cfg);
+    write_vtu_host(*cfg, make_mesh(mesh_desc(*cfg)), vec, t, filename);
+  });
+}
+
+int
 dg_host_index_map(const dg_config *cfg, int64_t *n_cells,
                   double *cell_centres, int64_t *dof_offsets)
 {

@@ -147,6 +147,13 @@ class AdvectionOperation:
                                       out))
         return tuple(out)
 
+    def write_vtu(self, vec, time, filename):
+        """VTU file with "solution" and "analytic_solution" at `time`, one
+        Lagrange cell of the operator's degree per cell, as output_results
+        writes it (advection_solver.cc:1217-1241)."""
+        L.check(self._lib.dg_write_vtu(self._h, self._v(vec, "vec"), time,
+                                       str(filename).encode()))
+
     # ---- implicit operator -------------------------------------------------
     def set_time(self, current_time, time_step):
         L.check(self._lib.dg_set_time(self._h, current_time, time_step))

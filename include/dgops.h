@@ -193,6 +193,18 @@ int dg_compute_mass_and_energy(dg_operator *op, const void *vec, double out[3]);
  * QGauss(k+1)                    output_results advection_solver.cc:1183-1215 */
 int dg_l2_error(dg_operator *op, const void *vec, double t, double out[2]);
 
+/* VTU file of a solution at time t: fields "solution" and
+ * "analytic_solution" (the exact solution interpolated into the DG space),
+ * one VTK Lagrange cell of the operator's degree per cell with points at
+ * equidistant reference positions (DataOut with write_higher_order_cells
+ * and build_patches(mapping, degree)); vec is a device vector of the
+ * operator's precision        output_results advection_solver.cc:1217-1241 */
+int dg_write_vtu(dg_operator *op, const void *vec, double t,
+                 const char *filename);
+/* the same file from a host vector (FP64, no GPU needed)                 */
+int dg_host_write_vtu(const dg_config *cfg, const double *vec, double t,
+                      const char *filename);
+
 /* ---- implicit single stage operator ----------------------------------- */
 
 /* advection_solver_implicit_real.cc:869-873                               */
