@@ -163,6 +163,26 @@ def test_projection_and_integrals_match_oracle(gpu_lib, name):
                        rtol=1e-12, atol=0)
 
 
+@pytest.mark.parametrize("name", ["3d_k4_pert_cellular", "3d_k4_cart_const",
+                                  "2d_k3_cart_const", "2d_k2_dirichlet"])
+def test_fp32_apply_error_against_fp64_oracle(gpu_lib, name):
+    """FP32 operator (tabulated and uniform geometry): the error stays
+    within a small multiple of the FP32 rounding of the input pushed
+    through the FP64 operator."""
+    c = Cfg(**CASES[name])
+    o = c.oracle_handle()
+    c.precision = 1
+    op = AdvectionOperation(c.dg())
+    u = np.random.default_rng(8).standard_normal(o.n_dofs)
+    dst = op.initialize_dof_vector()
+    op.apply(dev(u).float(), dst, 0.1)
+    ref = o.apply(u, 0.1)
+    floor = rel_diff(o.apply(u.astype(np.float32).astype(np.float64), 0.1),
+                     ref)
+    err = rel_diff(dst.double().cpu().numpy(), ref)
+    assert err < max(20 * floor, 2e-6), (err, floor)
+
+
 def test_errors_are_reported(gpu_lib):
     c, op, o = make("3d_k4_cart_const")
     u = op.initialize_dof_vector()

@@ -49,8 +49,9 @@ def test_vmult_matches_oracle(gpu_lib, name):
         assert rel_diff(host(dst), o.vmult(u, t, dt)) < TOL
 
 
-def test_vmult_fp32_error_against_fp64_oracle(gpu_lib):
-    c = Cfg(**IMPL["3d_k4_pert_cellular"])
+@pytest.mark.parametrize("name", ["3d_k4_pert_cellular", "3d_k3_cart"])
+def test_vmult_fp32_error_against_fp64_oracle(gpu_lib, name):
+    c = Cfg(**IMPL[name])
     o = c.oracle_handle()
     c.precision = 1
     op = AdvectionOperation(c.dg())
