@@ -4,6 +4,7 @@
 // are plain streaming kernels; the hot operator lives in op_kernel.hpp.
 #include "aux.hpp"
 #include "op_kernel.hpp"
+#include "registry.hpp"
 
 namespace dgops
 {
@@ -296,65 +297,6 @@ namespace dgops
       if (threadIdx.x == 0)
         for (int d = 0; d < dim; ++d)
           velocity[cell * dim + d] = tot[d] / v;
-    }
-
-    template <int dim, typename T>
-    __global__ void
-    inverse_mass_kernel(const OpParams P, const DevBasis<double> *__restrict__ B,
-                        int n, T *vec)
-    {
-      __shared__ double a[max_pts], b[max_pts];
-      __shared__ double X[24];
-      constexpr int     NV   = 1 << dim;
-      const int64_t     cell = blockIdx.x;
-      const int         nd   = dim == 3 ? n * n * n : n * n;
-      int               ext[3] = {n, n, dim == 3 ? n : 1};
-      // transposed inverse shape matrix for the S^-T sweeps
-      __shared__ double SinvT[max_n * max_n];
-      for (int e = threadIdx.x; e < n * n; e += blockDim.x)
-        SinvT[(e % n) * max_n + e / n] = B->Sinv[e / n][e % n];
-      for (int e = threadIdx.x; e < NV * dim; e += blockDim.x)
-        X[e] = P.cell_vertices[cell * NV * dim + e];
-      for (int pt = threadIdx.x; pt < nd; pt += blockDim.x)
-        a[pt] = (double)vec[cell * nd + pt];
-      __syncthreads();
-      sweep(SinvT, max_n, a, b, 0, n, n, ext);
-      __syncthreads();
-      sweep(SinvT, max_n, b, a, 1, n, n, ext);
-      __syncthreads();
-      double *q = a, *o = b;
-      if (dim == 3)
-        {
-          sweep(SinvT, max_n, a, b, 2, n, n, ext);
-          __syncthreads();
-          q = b;
-          o = a;
-        }
-      for (int pt = threadIdx.x; pt < nd; pt += blockDim.x)
-        {
-          const int qi[3] = {pt % n, (pt / n) % n, dim == 3 ? pt / (n * n) : 0};
-          double xi[3] = {B->xq[qi[0]], B->xq[qi[1]], dim == 3 ? B->xq[qi[2]] : 0.};
-          double x[3], J[3][3], adj[3][3];
-          multilinear_map<dim>(X, xi, x, J);
-          const double det = adjugate<dim>(J, adj);
-          const double w =
-            B->w[qi[0]] * B->w[qi[1]] * (dim == 3 ? B->w[qi[2]] : 1.);
-          q[pt] /= det * w;
-        }
-      __syncthreads();
-      sweep(&B->Sinv[0][0], max_n, q, o, 0, n, n, ext);
-      __syncthreads();
-      sweep(&B->Sinv[0][0], max_n, o, q, 1, n, n, ext);
-      __syncthreads();
-      double *res = q;
-      if (dim == 3)
-        {
-          sweep(&B->Sinv[0][0], max_n, q, o, 2, n, n, ext);
-          __syncthreads();
-          res = o;
-        }
-      for (int pt = threadIdx.x; pt < nd; pt += blockDim.x)
-        vec[cell * nd + pt] = (T)res[pt];
     }
 
     // sum of `k` interleaved columns of partials[count][k]
@@ -885,21 +827,13 @@ namespace dgops
     const OpParams P = op.base_params(0, 1);
     if (op.mesh.n_cells == 0)
       return;
-    const auto *B = op.d_basis_d.as<DevBasis<double>>();
-    const dim3  g(op.mesh.n_cells), b(aux_threads);
-    if (op.cfg.dim == 3 && p == DG_FP64)
-      hipLaunchKernelGGL((inverse_mass_kernel<3, double>), g, b, 0, op.stream,
-                         P, B, op.n, (double *)vec);
-    else if (op.cfg.dim == 3)
-      hipLaunchKernelGGL((inverse_mass_kernel<3, float>), g, b, 0, op.stream,
-                         P, B, op.n, (float *)vec);
-    else if (p == DG_FP64)
-      hipLaunchKernelGGL((inverse_mass_kernel<2, double>), g, b, 0, op.stream,
-                         P, B, op.n, (double *)vec);
-    else
-      hipLaunchKernelGGL((inverse_mass_kernel<2, float>), g, b, 0, op.stream,
-                         P, B, op.n, (float *)vec);
-    hip_check(hipGetLastError(), "inverse mass");
+    const int fp32 = p == DG_FP32 ? 1 : 0;
+    auto      it   = mass_registry().find(mass_key(op.cfg.dim, op.n, fp32));
+    if (it == mass_registry().end())
+      throw Error(DG_ERR_UNSUPPORTED, "inverse mass variant not built");
+    hip_check(it->second(P, op.basis_ptr(fp32 != 0), op.d_basis_d.ptr, vec,
+                         op.stream),
+              "inverse mass");
   }
 
   void

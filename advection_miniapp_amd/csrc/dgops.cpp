@@ -17,6 +17,13 @@ namespace dgops
     return m;
   }
 
+  std::map<uint32_t, MassLaunch> &
+  mass_registry()
+  {
+    static std::map<uint32_t, MassLaunch> m;
+    return m;
+  }
+
   std::map<uint32_t, BurgersLaunch> &
   burgers_registry()
   {

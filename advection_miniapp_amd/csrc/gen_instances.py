@@ -68,6 +68,12 @@ def burgers_variants():
     return v
 
 
+def mass_variants():
+    # cellwise inverse mass matrix, fp64 and fp32
+    return [("mass", dim, n, fp32) for dim, n, fp32 in
+            itertools.product((2, 3), range(2, 9), (0, 1))]
+
+
 def batch_variants():
     # serving loop: batched explicit LSRK stages, 3D, degrees 1..7
     return [("batch", 3, n) for n in range(2, 9)]
@@ -85,6 +91,13 @@ def emit(path, variants):
             lines.append(
                 f"  static BatchRegistrar s{i}(batch_key({dim}, {n}), "
                 f"&launch_batch<{dim}, {n}>);")
+            continue
+        if var[0] == "mass":
+            _, dim, n, fp32 = var
+            T = "float" if fp32 else "double"
+            lines.append(
+                f"  static MassRegistrar m{i}(mass_key({dim}, {n}, {fp32}), "
+                f"&launch_mass<{dim}, {n}, {T}>);")
             continue
         if var[0] == "burgers":
             _, dim, n, m, cart, epi = var
@@ -121,11 +134,11 @@ def main(n_files=24):
             return v[2] ** (v[1] + 1) * v[3]
         if v[0] == "burgers":
             return v[3] ** (v[1] + 1)
-        if v[0] == "batch":
+        if v[0] in ("batch", "mass"):
             return v[2] ** (v[1] + 1)
         return v[1] ** (v[0] + 1) * v[2] * (2 if v[6] else 1)
     variants = (op_variants() + fdm_variants() + burgers_variants() +
-                batch_variants())
+                batch_variants() + mass_variants())
     # spread heavy (3D, large n) variants evenly
     variants.sort(key=lambda v: -weight(v))
     groups = [[] for _ in range(n_files)]

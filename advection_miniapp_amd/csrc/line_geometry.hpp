@@ -388,4 +388,51 @@ namespace dgops
         jxw[i] = (T)(wl * wq[i] * det);
       }
   }
+
+  // 1/JxW at the Gauss points of line p of direction dir, as line_jxw.
+  // The reciprocal of det J is the hardware estimate refined by two
+  // Newton steps (relative error ~1 ulp; det J > 0 and well scaled on a
+  // valid mesh).
+  template <int dim, int n, int dir, typename T, typename Layout>
+  __device__ __forceinline__ void
+  line_inv_jxw(const double *X, const double *xq, const double *wq,
+               const double *winv, int p, T (&ij)[n])
+  {
+    int q[3];
+    Layout::template idx<dir>(p, 0, q);
+    double xi[3] = {xq[q[0]], xq[q[1]], dim == 3 ? xq[q[2]] : 0.};
+    double wl    = 1.;
+#pragma unroll
+    for (int e = 0; e < dim; ++e)
+      if (e != dir)
+        wl *= wq[q[e]];
+    const double rwl = 1. / wl;
+    double       x[3], J0[3][3], J1[3][3];
+    xi[dir] = 0.;
+    multilinear_map<dim>(X, xi, x, J0);
+    xi[dir] = 1.;
+    multilinear_map<dim>(X, xi, x, J1);
+#pragma unroll
+    for (int i = 0; i < n; ++i)
+      {
+        const double t = xq[i];
+        double       J[3][3];
+#pragma unroll
+        for (int a = 0; a < dim; ++a)
+#pragma unroll
+          for (int b = 0; b < dim; ++b)
+            J[a][b] = fma(t, J1[a][b] - J0[a][b], J0[a][b]);
+        double det;
+        if constexpr (dim == 3)
+          det = J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) +
+                J[1][0] * (J[2][1] * J[0][2] - J[0][1] * J[2][2]) +
+                J[2][0] * (J[0][1] * J[1][2] - J[1][1] * J[0][2]);
+        else
+          det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        double r = __builtin_amdgcn_rcp(det);
+        r        = fma(r, fma(-det, r, 1.), r);
+        r        = fma(r, fma(-det, r, 1.), r);
+        ij[i]    = (T)(rwl * winv[i] * r);
+      }
+  }
 } // namespace dgops

@@ -10,6 +10,7 @@
 
 #include "burgers_kernel.hpp"
 #include "fdm_kernel.hpp"
+#include "mass_kernel.hpp"
 #include "op_kernel.hpp"
 
 namespace dgops
@@ -128,6 +129,42 @@ namespace dgops
                        (const DevBasis<double> *)bd,
                        (const BurgersTables *)tables,
                        *(const OpBuffers<double> *)io);
+    return hipGetLastError();
+  }
+
+  // cellwise inverse mass matrix variants (in place)
+  using MassLaunch = hipError_t (*)(const OpParams &, const void *basis_t,
+                                    const void *basis_d, void *vec,
+                                    hipStream_t);
+
+  inline uint32_t
+  mass_key(int dim, int n, int fp32)
+  {
+    return (uint32_t)dim | (uint32_t)n << 2 | (uint32_t)fp32 << 6 | 1u << 22;
+  }
+
+  std::map<uint32_t, MassLaunch> &mass_registry();
+
+  struct MassRegistrar
+  {
+    MassRegistrar(uint32_t key, MassLaunch f) { mass_registry()[key] = f; }
+  };
+
+  template <int dim, int n, typename T>
+  hipError_t
+  launch_mass(const OpParams &P, const void *bt, const void *bd, void *vec,
+              hipStream_t stream)
+  {
+    constexpr int CPB    = cells_per_block<dim, n>();
+    constexpr int nl     = Lay<dim, n>::nl;
+    static_assert(CPB * nl <= 128, "kernels are bounded to 128 threads");
+    const int64_t blocks = (P.n_cells + CPB - 1) / CPB;
+    if (blocks <= 0)
+      return hipSuccess;
+    hipLaunchKernelGGL((inverse_mass_line_kernel<dim, n, T>),
+                       dim3((unsigned)blocks), dim3(CPB * nl), 0, stream, P,
+                       (const DevBasis<T> *)bt, (const DevBasis<double> *)bd,
+                       (T *)vec);
     return hipGetLastError();
   }
 

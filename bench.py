@@ -288,6 +288,106 @@ def time_one_apply(cfg_kw, steps=20, src=None):
     return op, src, dst, start.elapsed_time(stop) / steps
 
 
+def _events():
+    import torch
+    return (torch.cuda.Event(enable_timing=True),
+            torch.cuda.Event(enable_timing=True))
+
+
+def time_config2_run(n_steps=200, courant=0.3):
+    """Configuration 2 as a run: 200 LSRK(5,4) time steps at Courant 0.3
+    with fused stages (dg_lsrk_time_step, three registers); per stage 32
+    B/DoF of vector traffic (read r and u, write u and r')."""
+    import torch
+    from advection_miniapp_amd import (AdvectionOperation, make_config,
+                                       explicit_time_step)
+    from advection_miniapp_amd import _lib as L
+    op = AdvectionOperation(make_config(**apply_workload()))
+    # |a| <= sqrt(3) for the cellular field
+    dt = explicit_time_step(op.min_vertex_distance, 3 ** 0.5, 4, courant,
+                            1e9)
+    u = op.initialize_dof_vector()
+    op.project_initial(u, 0.0)
+    ri, ki = op.initialize_dof_vector(), op.initialize_dof_vector()
+    m0 = op.compute_mass_and_energy(u)
+    op._lsrk_time_step(L.DG_LSRK_5_4, 0.0, dt, u, ri, ki)
+    torch.cuda.synchronize()
+    start, stop = _events()
+    start.record()
+    for s in range(1, n_steps):
+        op._lsrk_time_step(L.DG_LSRK_5_4, s * dt, dt, u, ri, ki)
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop)
+    steps = n_steps - 1
+    stage_ms = ms / (5 * steps)
+    m1 = op.compute_mass_and_energy(u)
+    n = op.n_dofs
+    return dict(
+        workload="configuration 2 run: 200 LSRK(5,4) steps, Courant 0.3, "
+                 "fused stages (199 timed after one warm-up step)",
+        time_step=dt, ms_per_step=ms / steps, ms_per_stage=stage_ms,
+        dof_per_s_per_stage=n / (stage_ms * 1e-3),
+        stage_gb_per_s=32 * n / (stage_ms * 1e-3) / 1e9,
+        stage_roofline_frac=32 * n / (stage_ms * 1e-3) / 8e12,
+        run_seconds_200_steps=ms / steps * n_steps / 1e3,
+        mass_drift=abs(m1[0] - m0[0]) / abs(m0[0]))
+
+
+def time_inverse_mass(steps=20):
+    """Standalone cellwise inverse mass matrix on configuration 2 (in
+    place, 16 B/DoF)."""
+    import torch
+    from advection_miniapp_amd import AdvectionOperation, make_config
+    op = AdvectionOperation(make_config(**apply_workload()))
+    u = op.initialize_dof_vector()
+    op.project_initial(u, 0.0)
+    for _ in range(3):
+        op.apply_inverse_mass_matrix(u)
+    torch.cuda.synchronize()
+    start, stop = _events()
+    start.record()
+    for _ in range(steps):
+        op.apply_inverse_mass_matrix(u)
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / steps
+    n = op.n_dofs
+    return dict(ms=ms, dof_per_s=n / (ms * 1e-3),
+                gb_per_s=16 * n / (ms * 1e-3) / 1e9)
+
+
+def time_config1_run():
+    """Configuration 1 end to end on the GPU: 2D, degree 3, 32^2 cells on
+    [-1,1]^2, velocity (1, 0.15), u0 = sin(pi x) sin(pi y), LSRK(5,4) to
+    t = 2 at Courant 0.4 (421 steps), relative L2 error at t = 2."""
+    import math
+    import torch
+    from advection_miniapp_amd import (AdvectionOperation, make_config,
+                                       explicit_time_step)
+    from advection_miniapp_amd import _lib as L
+    op = AdvectionOperation(make_config(
+        dim=2, degree=3, refine=5, lo=[-1, -1, 0], hi=[1, 1, 1],
+        velocity=[1.0, 0.15, 0.0], solution_kind=2,
+        solution_param=[math.pi, 0.0], periodic=[1, 1, 1]))
+    dt = explicit_time_step(op.min_vertex_distance, math.hypot(1.0, 0.15),
+                            3, 0.4, 2.0)
+    n = int(round(2.0 / dt))
+    u = op.initialize_dof_vector()
+    ri, ki = op.initialize_dof_vector(), op.initialize_dof_vector()
+    torch.cuda.synchronize()
+    start, stop = _events()
+    start.record()
+    op.project_initial(u, 0.0)
+    for s in range(n):
+        op._lsrk_time_step(L.DG_LSRK_5_4, s * dt, dt, u, ri, ki)
+    stop.record()
+    torch.cuda.synchronize()
+    err, nrm = op.l2_error(u, 2.0)
+    return dict(steps=n, seconds=start.elapsed_time(stop) / 1e3,
+                rel_l2_error=err / nrm)
+
+
 def time_extra():
     """Configuration 3 (advection + SIPG diffusion, 32^3 cells, degrees
     3/5/7, FP64 and FP32 operators, FP32 error against FP64) and the
@@ -314,6 +414,9 @@ def time_extra():
             fp32_rel_err_vs_fp64=err)
         del src, dst64, dst32
         torch.cuda.empty_cache()
+    out["config2_run"] = time_config2_run()
+    out["inverse_mass"] = time_inverse_mass()
+    out["config1_run"] = time_config1_run()
     bk = dict(dim=3, degree=3, n_components=3, operator_kind=2,
               n_coarse=[3, 3, 3], refine=4, lo=[-1, -1, -1], hi=[1, 1, 1],
               solution_kind=6)

@@ -63,13 +63,26 @@ def test_vmult_fp32_error_against_fp64_oracle(gpu_lib, name):
     assert err < 5e-6, err
 
 
-def test_inverse_mass_matches_oracle(gpu_lib):
-    c = Cfg(**IMPL["3d_k4_pert_cellular"])
+@pytest.mark.parametrize("name", ["3d_k4_pert_cellular", "2d_k4_vortex_pert",
+                                  "3d_k2_dirichlet", "3d_k3_cart"])
+def test_inverse_mass_matches_oracle(gpu_lib, name):
+    c = Cfg(**IMPL[name])
     op, o = AdvectionOperation(c.dg()), c.oracle_handle()
     r = np.random.default_rng(5).standard_normal(o.n_dofs)
     d = dev(r)
     op.apply_inverse_mass_matrix(d)
     assert rel_diff(host(d), o.inverse_mass(r)) < TOL
+
+
+def test_inverse_mass_fp32_error(gpu_lib):
+    c = Cfg(**IMPL["3d_k4_pert_cellular"])
+    o = c.oracle_handle()
+    c.precision = 1
+    op = AdvectionOperation(c.dg())
+    r = np.random.default_rng(5).standard_normal(o.n_dofs)
+    d = dev(r, torch.float32)
+    op.apply_inverse_mass_matrix(d)
+    assert rel_diff(host(d), o.inverse_mass(r)) < 5e-6
 
 
 @pytest.mark.parametrize("s", [2, 3])
