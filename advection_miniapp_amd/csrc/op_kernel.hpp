@@ -558,10 +558,9 @@ namespace dgops
     // tabulated geometry: LAST-direction speed on this thread's line, the
     // y and x speed rows, and a . n dA on the faces its lines end on (lower
     // face of this cell, upper face = lower face of the upper neighbour,
-    // boundary faces apart). Several components: requested here, ahead of
-    // the barriers of the value phase, so that their latency overlaps it
-    // (with one component the registers this holds cost more occupancy
-    // than the overlap gains)
+    // boundary faces apart). Requested after the value phase: holding them
+    // across its barriers costs an occupancy step, which loses more than
+    // the overlap gains, with one component and with several.
     T cv[3][n], jxw[n], cvy[n], cvx[n];
     T fan[2 * dim];
     auto load_tables = [&]() {
@@ -598,12 +597,6 @@ namespace dgops
         table_line<dim, n, 1>(tab_c, cell, 1, p, active, cvy);
       table_line<dim, n, 0>(tab_c, cell, 0, p, active, cvx);
     };
-    constexpr bool early_tables = geo == geo_table && NC > 1;
-    if constexpr (early_tables)
-      {
-        load_tables();
-        load_speed_rows();
-      }
 
     // ---- per component: neighbour traces and values at quadrature points
     T uplus[NC][2 * dim];
@@ -874,8 +867,7 @@ namespace dgops
       }
     else if constexpr (!AN)
       {
-        if constexpr (!early_tables)
-          load_tables();
+        load_tables();
         // JxW of the mass term on this thread's line
         if (fin == fin_integrate && active && P.has_mass)
           line_jxw<dim, n, LAST, T, L>(X, Bd->xq, Bd->w, p, jxw);
@@ -920,7 +912,7 @@ namespace dgops
     }
 
     // speed rows of the y and x phases, read once for all components
-    if constexpr (geo == geo_table && !early_tables)
+    if constexpr (geo == geo_table)
       load_speed_rows();
     if constexpr (UNI)
       {
