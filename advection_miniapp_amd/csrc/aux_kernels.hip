@@ -700,7 +700,7 @@ namespace dgops
           const int q[3] = {pt % n, (pt / n) % n,
                             dim == 3 ? pt / (n * n) : 0};
           double    cv[3], jxw;
-          cell_point_data<dim, false>(P, X, B->xq, B->w, q, cv, jxw);
+          cell_point_data<dim>(P, X, B->xq, B->w, q, cv, jxw);
           for (int d = 0; d < dim; ++d)
             tc[d * nd + pt] = cv[d];
           tc[dim * nd + pt] = 1. / jxw;
@@ -711,7 +711,7 @@ namespace dgops
           int       q[3];
           face_line_point<dim, n>(d, p, q);
           double an, x[3];
-          face_point_data<dim, false>(P, X, B->xq, B->w, d, 0, q, an, x);
+          face_point_data<dim>(P, X, B->xq, B->w, d, 0, q, an, x);
           tab_face[(cell * dim + d) * nl + p] = -an;
         }
     }
@@ -737,7 +737,7 @@ namespace dgops
           int q[3];
           face_line_point<dim, n>(f / 2, p, q);
           double an, x[3] = {0, 0, 0};
-          face_point_data<dim, false>(P, X, B->xq, B->w, f / 2, f % 2, q, an,
+          face_point_data<dim>(P, X, B->xq, B->w, f / 2, f % 2, q, an,
                                       x);
           double *r = tab + (b * nl + p) * (1 + dim);
           r[0]      = an;

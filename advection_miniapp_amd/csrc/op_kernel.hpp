@@ -233,8 +233,9 @@ namespace dgops
   // Per quadrature point data of the cell term: cv[d] = w * (adj(J) a)_d
   // (the reference-space transport speed times JxW, which is what
   // advection_solver_implicit_real.cc:1126-1128 and :998-1007 fold
-  // together) and JxW = w det(J).
-  template <int dim, bool cartesian>
+  // together) and JxW = w det(J), on a multilinear cell (the geometry
+  // table builder).
+  template <int dim>
   __device__ __forceinline__ void
   cell_point_data(const OpParams &P,
                   const double   *X,
@@ -247,40 +248,26 @@ namespace dgops
     double w = wq[q[0]] * wq[q[1]];
     if (dim == 3)
       w *= wq[q[2]];
-    double x[3], a[3] = {0, 0, 0};
-    if constexpr (cartesian)
+    double       x[3], a[3] = {0, 0, 0};
+    const double xi[3] = {xq[q[0]], xq[q[1]], dim == 3 ? xq[q[2]] : 0.};
+    double       J[3][3], adj[3][3];
+    multilinear_map<dim>(X, xi, x, J);
+    velocity_field<dim>(P.vel_kind, P.vel_const, x, a);
+    const double det = adjugate<dim>(J, adj);
+    for (int d = 0; d < dim; ++d)
       {
-        for (int d = 0; d < dim; ++d)
-          x[d] = X[d] + P.h[d] * xq[q[d]];
-        velocity_field<dim>(P.vel_kind, P.vel_const, x, a);
-        double vol = P.h[0] * P.h[1];
-        if (dim == 3)
-          vol *= P.h[2];
-        for (int d = 0; d < dim; ++d)
-          cv[d] = w * (vol / P.h[d]) * a[d];
-        JxW = w * vol;
+        double s = 0;
+        for (int e = 0; e < dim; ++e)
+          s += adj[d][e] * a[e];
+        cv[d] = w * s;
       }
-    else
-      {
-        const double xi[3] = {xq[q[0]], xq[q[1]], dim == 3 ? xq[q[2]] : 0.};
-        double       J[3][3], adj[3][3];
-        multilinear_map<dim>(X, xi, x, J);
-        velocity_field<dim>(P.vel_kind, P.vel_const, x, a);
-        const double det = adjugate<dim>(J, adj);
-        for (int d = 0; d < dim; ++d)
-          {
-            double s = 0;
-            for (int e = 0; e < dim; ++e)
-              s += adj[d][e] * a[e];
-            cv[d] = w * s;
-          }
-        JxW = w * det;
-      }
+    JxW = w * det;
   }
 
   // Face point data: an = a . n dA (outward normal of this cell, including
-  // the face quadrature weight) and the physical point x.
-  template <int dim, bool cartesian>
+  // the face quadrature weight) and the physical point x, on a multilinear
+  // cell (the face and boundary table builders).
+  template <int dim>
   __device__ __forceinline__ void
   face_point_data(const OpParams &P,
                   const double   *X,
@@ -301,29 +288,14 @@ namespace dgops
       xi[e] = (e < dim) ? xq[q[e]] : 0.;
     xi[dir] = side ? 1. : 0.;
     double a[3] = {0, 0, 0};
-    if constexpr (cartesian)
-      {
-        for (int d = 0; d < dim; ++d)
-          x[d] = X[d] + P.h[d] * xi[d];
-        x[2] = dim == 3 ? x[2] : 0.;
-        velocity_field<dim>(P.vel_kind, P.vel_const, x, a);
-        double area = 1.;
-        for (int e = 0; e < dim; ++e)
-          if (e != dir)
-            area *= P.h[e];
-        an = (side ? 1. : -1.) * w * area * a[dir];
-      }
-    else
-      {
-        double J[3][3], adj[3][3];
-        multilinear_map<dim>(X, xi, x, J);
-        velocity_field<dim>(P.vel_kind, P.vel_const, x, a);
-        adjugate<dim>(J, adj);
-        double s = 0;
-        for (int e = 0; e < dim; ++e)
-          s += adj[dir][e] * a[e];
-        an = (side ? 1. : -1.) * w * s;
-      }
+    double J[3][3], adj[3][3];
+    multilinear_map<dim>(X, xi, x, J);
+    velocity_field<dim>(P.vel_kind, P.vel_const, x, a);
+    adjugate<dim>(J, adj);
+    double s = 0;
+    for (int e = 0; e < dim; ++e)
+      s += adj[dir][e] * a[e];
+    an = (side ? 1. : -1.) * w * s;
   }
 
 
@@ -335,12 +307,11 @@ namespace dgops
   // advection_solver_implicit_real.cc:1032-1058:
   //   F = 1/2 [(u- + u+) an + alpha |an| (u- - u+)] - skew an u-
   // with an = a . n dA on this cell's outward normal.
-  template <int dim, int n, int dir, typename T, int geo, bool diff>
+  template <int dim, int n, int dir, typename T, bool diff>
   __device__ __forceinline__ void
   face_terms(const OpParams &P,
              const DevBasis<double> *Bd,
              const DevBasis<T>      &B,
-             const double           *X,
              int64_t                 cell,
              int                     p,
              bool                    active,
@@ -350,8 +321,7 @@ namespace dgops
              const T (&uq)[n],
              T (&acc)[n],
              T    vf,
-             T (&fan)[2 * dim],
-             bool compute_geometry)
+             T (&fan)[2 * dim])
   {
     int q[3];
     Lay<dim, n>::template idx<dir>(p, 0, q);
@@ -961,11 +931,9 @@ namespace dgops
             }
           if constexpr (diff)
             diffusion_cell_term<dim, n, LAST, T>(P, Bd, B, p, uqs[c], acc);
-          face_terms<dim, n, LAST, T, geo, diff>(P, Bd, B, X, cell, p,
-                                                       active,
-                                                       nb, uplus[c], dplus[c],
-                                                       uqs[c], acc, vf, fan,
-                                                       c == 0);
+          face_terms<dim, n, LAST, T, diff>(P, Bd, B, cell, p, active, nb,
+                                            uplus[c], dplus[c], uqs[c], acc,
+                                            vf, fan);
           lds_write_line<dim, n, LAST>(u_l, p, uqs[c]);
           lds_write_line<dim, n, LAST>(acc_l, p, acc);
           if constexpr (AN)
@@ -1031,11 +999,9 @@ namespace dgops
               }
             if constexpr (diff)
               diffusion_cell_term<dim, n, 1, T>(P, Bd, B, p, uq, acc);
-            face_terms<dim, n, 1, T, geo, diff>(P, Bd, B, X, cell, p,
-                                                      active,
-                                                      nb, uplus[c], dplus[c],
-                                                      uq, acc, vf, fan,
-                                                      c == 0);
+            face_terms<dim, n, 1, T, diff>(P, Bd, B, cell, p, active, nb,
+                                           uplus[c], dplus[c], uq, acc, vf,
+                                           fan);
             lds_write_line<dim, n, 1>(acc_l, p, acc);
             __syncthreads();
           }
@@ -1071,10 +1037,9 @@ namespace dgops
             }
           if constexpr (diff)
             diffusion_cell_term<dim, n, 0, T>(P, Bd, B, p, uq, acc);
-          face_terms<dim, n, 0, T, geo, diff>(P, Bd, B, X, cell, p,
-                                                    active,
-                                                    nb, uplus[c], dplus[c], uq,
-                                                    acc, vf, fan, c == 0);
+          face_terms<dim, n, 0, T, diff>(P, Bd, B, cell, p, active, nb,
+                                         uplus[c], dplus[c], uq, acc, vf,
+                                         fan);
         }
         __syncthreads();
         if constexpr (fin == fin_inverse_mass)
