@@ -507,6 +507,10 @@ namespace dgops
       for (int i = 0; i < n; ++i)
         r[i] = active ? src[cell * nd + L::template off<0>(p, i)] : (T)0;
     };
+    // waves that are issuing global loads (the cell's values and traces,
+    // the tables) or stores get issue priority, so that requests leave
+    // early while other waves compute
+    __builtin_amdgcn_s_setprio(3);
     T r0[n];
     load_own(io.src, r0);
     // cell vertices: analytic geometry, and JxW of the mass term with
@@ -608,6 +612,7 @@ namespace dgops
               }
           }
         // own values, sweep along x
+        __builtin_amdgcn_s_setprio(0);
         mat_apply<n>(B.S, r, t);
         lds_write_line<dim, n, 0>(acc_l, p, t);
         if constexpr (dim == 3 && !diff)
@@ -837,7 +842,9 @@ namespace dgops
       }
     else if constexpr (!AN)
       {
+        __builtin_amdgcn_s_setprio(3);
         load_tables();
+        __builtin_amdgcn_s_setprio(0);
         // JxW of the mass term on this thread's line
         if (fin == fin_integrate && active && P.has_mass)
           line_jxw<dim, n, LAST, T, L>(X, Bd->xq, Bd->w, p, jxw);
@@ -1097,6 +1104,7 @@ namespace dgops
         }
 
         // ---- epilogue on LAST-direction lines (coalesced) ----
+        __builtin_amdgcn_s_setprio(3);
         if (active)
           {
             const int64_t base = c * io.comp_stride + cell * nd;
@@ -1121,7 +1129,10 @@ namespace dgops
               }
           }
         if (NC > 1)
-          __syncthreads();
+          {
+            __builtin_amdgcn_s_setprio(0);
+            __syncthreads();
+          }
       }
   }
 
