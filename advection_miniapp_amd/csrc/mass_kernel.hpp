@@ -47,18 +47,18 @@ namespace dgops
         X[e] = P.cell_vertices[cell * NV * dim + e];
 
     // S^-T sweeps: x (registers), then y (and z) through LDS
-    mat_apply_t<n>(B.Sinv, r, t);
+    eo_apply<n, false, true, false>(B.eo_Sinv, r, t);
     lds_write_line<dim, n, 0>(a_l, p, t);
     __syncthreads();
     lds_read_line<dim, n, 1>(a_l, p, t);
-    mat_apply_t<n>(B.Sinv, t, r);
+    eo_apply<n, false, true, false>(B.eo_Sinv, t, r);
     if constexpr (dim == 3)
       {
         __syncthreads();
         lds_write_line<dim, n, 1>(a_l, p, r);
         __syncthreads();
         lds_read_line<dim, n, 2>(a_l, p, t);
-        mat_apply_t<n>(B.Sinv, t, r);
+        eo_apply<n, false, true, false>(B.eo_Sinv, t, r);
       }
 
     // 1/JxW at the Gauss points of this thread's LAST-direction line
@@ -74,20 +74,20 @@ namespace dgops
       r[i] *= ij[i];
 
     // S^-1 sweeps back: LAST (registers), then through LDS to x
-    mat_apply<n>(B.Sinv, r, t);
+    eo_apply<n, false, false, false>(B.eo_Sinv, r, t);
     if constexpr (dim == 3)
       {
         __syncthreads();
         lds_write_line<dim, n, 2>(a_l, p, t);
         __syncthreads();
         lds_read_line<dim, n, 1>(a_l, p, r);
-        mat_apply<n>(B.Sinv, r, t);
+        eo_apply<n, false, false, false>(B.eo_Sinv, r, t);
       }
     __syncthreads();
     lds_write_line<dim, n, 1>(a_l, p, t);
     __syncthreads();
     lds_read_line<dim, n, 0>(a_l, p, r);
-    mat_apply<n>(B.Sinv, r, t);
+    eo_apply<n, false, false, false>(B.eo_Sinv, r, t);
     if (active)
 #pragma unroll
       for (int i = 0; i < n; ++i)

@@ -196,6 +196,83 @@ namespace dgops
       }
   }
 
+  // Even-odd products with a centrosymmetric (skew = false) or
+  // skew-centrosymmetric (skew = true) matrix M (EvenOdd in dev_basis.hpp):
+  // out = M in, or M^T in (transpose), added to out (add). With
+  // e_i = in_i + in_{n-1-i}, o_i = in_i - in_{n-1-i}, rows q and n-1-q share
+  //   se = sum_i E[q][i] e_i + M[q][m] in_m,   so = sum_i O[q][i] o_i
+  // as out_q = se + so and out_{n-1-q} = se - so (so - se when skew); the
+  // middle row is sum_i M[m][i] e_i + M[m][m] in_m (sum_i M[m][i] o_i when
+  // skew). M^T has the same form with E, O transposed (and exchanged when
+  // skew) and the middle row and column exchanged.
+  template <int n, bool skew, bool transpose, bool add, typename T>
+  __device__ __forceinline__ void
+  eo_apply(const EvenOdd<T> &M, const T (&in)[n], T (&out)[n])
+  {
+    constexpr int  h   = n / 2;
+    constexpr int  m   = h;
+    constexpr bool odd = n % 2 == 1;
+    T              e[h > 0 ? h : 1], o[h > 0 ? h : 1];
+#pragma unroll
+    for (int i = 0; i < h; ++i)
+      {
+        e[i] = in[i] + in[n - 1 - i];
+        o[i] = in[i] - in[n - 1 - i];
+      }
+    auto E = [&](int q, int i) -> T {
+      if constexpr (!transpose)
+        return M.e[q][i];
+      else if constexpr (!skew)
+        return M.e[i][q];
+      else
+        return M.o[i][q];
+    };
+    auto O = [&](int q, int i) -> T {
+      if constexpr (!transpose)
+        return M.o[q][i];
+      else if constexpr (!skew)
+        return M.o[i][q];
+      else
+        return M.e[i][q];
+    };
+    auto col = [&](int q) -> T { return transpose ? M.mr[q] : M.mc[q]; };
+    auto row = [&](int i) -> T { return transpose ? M.mc[i] : M.mr[i]; };
+#pragma unroll
+    for (int q = 0; q < h; ++q)
+      {
+        T se = odd ? col(q) * in[m] : E(q, 0) * e[0];
+#pragma unroll
+        for (int i = odd ? 0 : 1; i < h; ++i)
+          se = fma(E(q, i), e[i], se);
+        T so = O(q, 0) * o[0];
+#pragma unroll
+        for (int i = 1; i < h; ++i)
+          so = fma(O(q, i), o[i], so);
+        const T lo = se + so, hi = skew ? so - se : se - so;
+        out[q]         = add ? out[q] + lo : lo;
+        out[n - 1 - q] = add ? out[n - 1 - q] + hi : hi;
+      }
+    if constexpr (odd)
+      {
+        T s;
+        if constexpr (skew)
+          {
+            s = h > 0 ? row(0) * o[0] : (T)0;
+#pragma unroll
+            for (int i = 1; i < h; ++i)
+              s = fma(row(i), o[i], s);
+          }
+        else
+          {
+            s = M.mm * in[m];
+#pragma unroll
+            for (int i = 0; i < h; ++i)
+              s = fma(row(i), e[i], s);
+          }
+        out[m] = add ? out[m] + s : s;
+      }
+  }
+
   template <int dim, int n, int dir, typename T>
   __device__ __forceinline__ void
   lds_write_line(T *buf, int p, const T (&v)[n])
@@ -394,7 +471,7 @@ namespace dgops
       vol *= P.h[2];
     const T c0 = (T)(P.sign * P.viscosity * vol / (P.h[dir] * P.h[dir]));
     T       du[n], g[n];
-    mat_apply<n>(B.D, uq, du);
+    eo_apply<n, true, false, false>(B.eo_D, uq, du);
 #pragma unroll
     for (int i = 0; i < n; ++i)
       {
@@ -405,7 +482,7 @@ namespace dgops
           w *= (T)Bd->w[q[2]];
         g[i] = c0 * w * du[i];
       }
-    mat_apply_t_add<n>(B.D, g, acc);
+    eo_apply<n, true, true, true>(B.eo_D, g, acc);
   }
 
   // ---- the kernel ------------------------------------------------------------
@@ -613,7 +690,7 @@ namespace dgops
           }
         // own values, sweep along x
         __builtin_amdgcn_s_setprio(0);
-        mat_apply<n>(B.S, r, t);
+        eo_apply<n, false, false, false>(B.eo_S, r, t);
         lds_write_line<dim, n, 0>(acc_l, p, t);
         if constexpr (dim == 3 && !diff)
           {
@@ -756,7 +833,7 @@ namespace dgops
               }
           }
         lds_read_line<dim, n, 1>(acc_l, p, t);
-        mat_apply<n>(B.S, t, r);
+        eo_apply<n, false, false, false>(B.eo_S, t, r);
         if constexpr (dim == 3)
           {
             __syncthreads();
@@ -784,7 +861,7 @@ namespace dgops
                   }
               }
             lds_read_line<dim, n, 2>(acc_l, p, t);
-            mat_apply<n>(B.S, t, r);
+            eo_apply<n, false, false, false>(B.eo_S, t, r);
           }
 #pragma unroll
         for (int i = 0; i < n; ++i)
@@ -927,11 +1004,11 @@ namespace dgops
 #pragma unroll
           for (int i = 0; i < n; ++i)
             g[i] = fcell * cv[LAST][i] * uqs[c][i];
-          mat_apply_t_add<n>(B.D, g, acc);
+          eo_apply<n, true, true, true>(B.eo_D, g, acc);
           if constexpr (skew)
             {
               T du[n];
-              mat_apply<n>(B.D, uqs[c], du);
+              eo_apply<n, true, false, false>(B.eo_D, uqs[c], du);
 #pragma unroll
               for (int i = 0; i < n; ++i)
                 acc[i] = fma(fs * cv[LAST][i], du[i], acc[i]);
@@ -989,11 +1066,11 @@ namespace dgops
                   g[i] = fcell * cvd[i] * uq[i];
               }
             lds_read_line<dim, n, 1>(acc_l, p, acc);
-            mat_apply_t_add<n>(B.D, g, acc);
+            eo_apply<n, true, true, true>(B.eo_D, g, acc);
             if constexpr (skew)
               {
                 T du[n], sk[n];
-                mat_apply<n>(B.D, uq, du);
+                eo_apply<n, true, false, false>(B.eo_D, uq, du);
                 if constexpr (AN)
                   lds_read_line<dim, n, 1>(sk1_l, p, sk);
                 else
@@ -1027,11 +1104,11 @@ namespace dgops
                 g[i] = fcell * cvd[i] * uq[i];
             }
           lds_read_line<dim, n, 0>(acc_l, p, acc);
-          mat_apply_t_add<n>(B.D, g, acc);
+          eo_apply<n, true, true, true>(B.eo_D, g, acc);
           if constexpr (skew)
             {
               T du[n], sk[n];
-              mat_apply<n>(B.D, uq, du);
+              eo_apply<n, true, false, false>(B.eo_D, uq, du);
               if constexpr (AN)
                 lds_read_line<dim, n, 0>(sk0_l, p, sk);
               else
@@ -1080,16 +1157,16 @@ namespace dgops
         {
           T t[n];
           if constexpr (fin == fin_inverse_mass)
-            mat_apply<n>(B.Sinv, acc, t);
+            eo_apply<n, false, false, false>(B.eo_Sinv, acc, t);
           else
-            mat_apply_t<n>(B.S, acc, t);
+            eo_apply<n, false, true, false>(B.eo_S, acc, t);
           lds_write_line<dim, n, 0>(acc_l, p, t);
           __syncthreads();
           lds_read_line<dim, n, 1>(acc_l, p, t);
           if constexpr (fin == fin_inverse_mass)
-            mat_apply<n>(B.Sinv, t, acc);
+            eo_apply<n, false, false, false>(B.eo_Sinv, t, acc);
           else
-            mat_apply_t<n>(B.S, t, acc);
+            eo_apply<n, false, true, false>(B.eo_S, t, acc);
           if constexpr (dim == 3)
             {
               __syncthreads();
@@ -1097,9 +1174,9 @@ namespace dgops
               __syncthreads();
               lds_read_line<dim, n, 2>(acc_l, p, t);
               if constexpr (fin == fin_inverse_mass)
-                mat_apply<n>(B.Sinv, t, acc);
+                eo_apply<n, false, false, false>(B.eo_Sinv, t, acc);
               else
-                mat_apply_t<n>(B.S, t, acc);
+                eo_apply<n, false, true, false>(B.eo_S, t, acc);
             }
         }
 
