@@ -294,6 +294,22 @@ def _events():
             torch.cuda.Event(enable_timing=True))
 
 
+def time_apply_by_degree():
+    """The configuration-2 apply (perturbed cells, cellular speed, FP64)
+    at degrees 1..7: 64^3 cells up to degree 4, 32^3 above."""
+    import torch
+    out = {}
+    for k in range(1, 8):
+        r = 6 if k <= 4 else 5
+        op, src, dst, ms = time_one_apply(apply_workload(refine=r, degree=k))
+        n = op.n_dofs
+        out[f"k{k}"] = dict(n_dofs=n, ms=ms, dof_per_s=n / (ms * 1e-3),
+                            roofline_frac=16 * n / (ms * 1e-3) / 8e12)
+        del op, src, dst
+        torch.cuda.empty_cache()
+    return out
+
+
 def time_config2_run(n_steps=200, courant=0.3):
     """Configuration 2 as a run: 200 LSRK(5,4) time steps at Courant 0.3
     with fused stages (dg_lsrk_time_step, three registers); per stage 32
@@ -414,6 +430,7 @@ def time_extra():
             fp32_rel_err_vs_fp64=err)
         del src, dst64, dst32
         torch.cuda.empty_cache()
+    out["apply_by_degree"] = time_apply_by_degree()
     out["config2_run"] = time_config2_run()
     out["inverse_mass"] = time_inverse_mass()
     out["config1_run"] = time_config1_run()
