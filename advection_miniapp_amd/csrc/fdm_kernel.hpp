@@ -174,7 +174,6 @@ namespace dgops
     constexpr int LAST = dim - 1;
 
     __shared__ Tc s_data[CPB * NS * STR];
-    __shared__ Tc s_t[CPB * STR];
 
     const FdmDev<Tc> &F    = *Fp;
     const int         tid  = threadIdx.x;
@@ -185,7 +184,6 @@ namespace dgops
     const bool        active = (lc < CPB) && (cell < P.n_cells);
     const int         lcs    = lc < CPB ? lc : 0;
     Tc               *data   = s_data + lcs * NS * STR;
-    Tc               *tb     = s_t + lcs * STR;
 
     bool rv[3]   = {false, false, false};
     Tc   aabs[3] = {0, 0, 0};
@@ -202,7 +200,10 @@ namespace dgops
       }
 
     // 1-2: load (x mirrored), stage transform, V^-1 M^-1 sweeps x, y, (z)
-    // with the y and z mirrors on the LDS reads
+    // with the y and z mirrors on the LDS reads. Every stage has its own
+    // LDS block; a thread rewrites exactly the positions of its own line
+    // (the mirrors permute within a line), so all stages share one barrier
+    // per direction change.
     {
       Tc r[NS][n];
 #pragma unroll
@@ -227,20 +228,30 @@ namespace dgops
               w[i] = NS == 1 ? r[0][i] : s;
             }
           mat_apply<n>(F.VinvMinv, w, t);
-          lds_write_line<dim, n, 0>(tb, p, t);
-          __syncthreads();
-          lds_read_line_m<dim, n, 1>(tb, p, rv[1], t);
-          mat_apply<n>(F.VinvMinv, t, w);
-          if constexpr (dim == 3)
-            {
-              lds_write_line<dim, n, 1>(tb, p, w);
-              __syncthreads();
-              lds_read_line_m<dim, n, 2>(tb, p, rv[2], t);
-              mat_apply<n>(F.VinvMinv, t, w);
-            }
-          lds_write_line<dim, n, LAST>(data + j * STR, p, w);
-          __syncthreads();
+          lds_write_line<dim, n, 0>(data + j * STR, p, t);
         }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < NS; ++j)
+        {
+          Tc w[n], t[n];
+          lds_read_line_m<dim, n, 1>(data + j * STR, p, rv[1], t);
+          mat_apply<n>(F.VinvMinv, t, w);
+          lds_write_line<dim, n, 1>(data + j * STR, p, w);
+        }
+      if constexpr (dim == 3)
+        {
+          __syncthreads();
+#pragma unroll
+          for (int j = 0; j < NS; ++j)
+            {
+              Tc w[n], t[n];
+              lds_read_line_m<dim, n, 2>(data + j * STR, p, rv[2], t);
+              mat_apply<n>(F.VinvMinv, t, w);
+              lds_write_line<dim, n, 2>(data + j * STR, p, w);
+            }
+        }
+      __syncthreads();
     }
 
     // 3: pointwise inverse on the blocks of this cell
@@ -319,20 +330,27 @@ namespace dgops
         Tc t[n];
         lds_read_line<dim, n, LAST>(data + j * STR, p, t);
         mat_apply<n>(F.V, t, w[j]);
-        if constexpr (dim == 3)
+        lds_write_line_m<dim, n, LAST>(data + j * STR, p, rv[LAST], w[j]);
+      }
+    __syncthreads();
+    if constexpr (dim == 3)
+      {
+#pragma unroll
+        for (int j = 0; j < NS; ++j)
           {
-            lds_write_line_m<dim, n, 2>(tb, p, rv[2], w[j]);
-            __syncthreads();
-            lds_read_line<dim, n, 1>(tb, p, t);
+            Tc t[n];
+            lds_read_line<dim, n, 1>(data + j * STR, p, t);
             mat_apply<n>(F.V, t, w[j]);
-            lds_write_line_m<dim, n, 1>(tb, p, rv[1], w[j]);
+            lds_write_line_m<dim, n, 1>(data + j * STR, p, rv[1], w[j]);
           }
-        else
-          lds_write_line_m<dim, n, 1>(tb, p, rv[1], w[j]);
         __syncthreads();
-        lds_read_line<dim, n, 0>(tb, p, t);
+      }
+#pragma unroll
+    for (int j = 0; j < NS; ++j)
+      {
+        Tc t[n];
+        lds_read_line<dim, n, 0>(data + j * STR, p, t);
         mat_apply<n>(F.V, t, w[j]);
-        __syncthreads();
       }
 
     // 5: back stage transform and store (x-lines, mirrored in x)
