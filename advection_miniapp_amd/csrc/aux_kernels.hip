@@ -701,9 +701,17 @@ namespace dgops
                             dim == 3 ? pt / (n * n) : 0};
           double    cv[3], jxw;
           cell_point_data<dim>(P, X, B->xq, B->w, q, cv, jxw);
+          // row d in the line order of direction d (row dim, read along x,
+          // in that of x): point i of line p at i * nl + p
           for (int d = 0; d < dim; ++d)
-            tc[d * nd + pt] = cv[d];
-          tc[dim * nd + pt] = 1. / jxw;
+            {
+              const int i = q[d];
+              const int p = d == 0 ? q[1] + n * q[2]
+                          : d == 1 ? q[0] + n * q[2]
+                                   : q[0] + n * q[1];
+              tc[d * nd + i * nl + p] = cv[d];
+            }
+          tc[dim * nd + q[0] * nl + q[1] + n * q[2]] = 1. / jxw;
         }
       for (int t = threadIdx.x; t < dim * nl; t += blockDim.x)
         {

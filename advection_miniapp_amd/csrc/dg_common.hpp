@@ -49,8 +49,10 @@ namespace dgops
     double        penalty;
     // geometry tables of the tabulated-geometry kernels (scalar type of
     // the kernel): per cell [dim + 1][nd] = w adj(J) a (dim rows) and
-    // 1/JxW at the Gauss points (JxW of the implicit mass term is
-    // recomputed from the corners along the lines); per
+    // 1/JxW at the Gauss points, each row in the line order of the
+    // direction it is read along (row d along d, row dim along x: point i
+    // of line p at i * nl + p; JxW of the implicit mass term is recomputed
+    // from the corners along the lines); per
     // cell [dim][nl] = w (adj(J) a)_d on the face at xi_d = 0 (normal
     // along +xi_d); per boundary face [nl][1 + dim] (double) = outward
     // a . n dA and the physical point

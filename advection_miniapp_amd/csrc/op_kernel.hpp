@@ -291,18 +291,21 @@ namespace dgops
       v[i] = buf[Lay<dim, n>::template off<dir>(p, i)];
   }
 
-  // row `row` of a cell's geometry table [dim + 1][nd], read along line p
-  // of direction dir
-  template <int dim, int n, int dir, typename T>
+  // row `row` of a cell's geometry table [dim + 1][nd] along line p of the
+  // direction the row is stored in (the direction it is read along: row d
+  // along d, row dim along x), point i at i * nl + p, so that the lanes of
+  // a cell read contiguous memory
+  template <int dim, int n, typename T>
   __device__ __forceinline__ void
   table_line(const T *tab, int64_t cell, int row, int p, bool active,
              T (&v)[n])
   {
     constexpr int nd   = Lay<dim, n>::nd;
+    constexpr int nl   = Lay<dim, n>::nl;
     const T      *base = tab + (cell * (dim + 1) + row) * nd;
 #pragma unroll
     for (int i = 0; i < n; ++i)
-      v[i] = active ? base[Lay<dim, n>::template off<dir>(p, i)] : (T)0;
+      v[i] = active ? base[i * nl + p] : (T)0;
   }
 
   // ---- geometry ------------------------------------------------------------
@@ -615,7 +618,7 @@ namespace dgops
     T cv[3][n], jxw[n], cvy[n], cvx[n];
     T fan[2 * dim];
     auto load_tables = [&]() {
-        table_line<dim, n, LAST>(tab_c, cell, LAST, p, active, cv[LAST]);
+        table_line<dim, n>(tab_c, cell, LAST, p, active, cv[LAST]);
 #pragma unroll
         for (int d = 0; d < dim; ++d)
           {
@@ -645,8 +648,8 @@ namespace dgops
     };
     auto load_speed_rows = [&]() {
       if constexpr (dim == 3)
-        table_line<dim, n, 1>(tab_c, cell, 1, p, active, cvy);
-      table_line<dim, n, 0>(tab_c, cell, 0, p, active, cvx);
+        table_line<dim, n>(tab_c, cell, 1, p, active, cvy);
+      table_line<dim, n>(tab_c, cell, 0, p, active, cvx);
     };
 
     // ---- per component: neighbour traces and values at quadrature points
@@ -1147,7 +1150,7 @@ namespace dgops
                   ij[i] = B.winv[i] * r;
               }
             else
-              table_line<dim, n, 0>(tab_c, cell, dim, p, active, ij);
+              table_line<dim, n>(tab_c, cell, dim, p, active, ij);
 #pragma unroll
             for (int i = 0; i < n; ++i)
               acc[i] *= ij[i];
