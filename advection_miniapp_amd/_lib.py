@@ -88,8 +88,8 @@ class DgServeRequest(ctypes.Structure):
     ]
 
 
-DG_REQUEST_QUEUED, DG_REQUEST_ACTIVE, DG_REQUEST_DONE, DG_REQUEST_UNKNOWN = \
-    range(4)
+(DG_REQUEST_QUEUED, DG_REQUEST_ACTIVE, DG_REQUEST_DONE, DG_REQUEST_UNKNOWN,
+ DG_REQUEST_FAILED) = range(5)
 
 _vp = ctypes.c_void_p
 _d = ctypes.c_double

@@ -166,14 +166,29 @@ namespace dgops
           if (used + need > budget)
             break;
           queue.pop_front();
-          dg_operator *h  = nullptr;
-          const int    rc = dg_create(&q.cfg, stream, &h);
-          if (rc != DG_OK)
-            throw Error(rc, dg_last_error());
-          q.handle = h;
-          for (auto &r : q.reg)
-            r.alloc(q.vec_bytes);
-          aux_project(*q.op(), q.reg[0].ptr, 0.);
+          try
+            {
+              dg_operator *h  = nullptr;
+              const int    rc = dg_create(&q.cfg, stream, &h);
+              if (rc != DG_OK)
+                throw Error(rc, dg_last_error());
+              q.handle = h;
+              for (auto &r : q.reg)
+                r.alloc(q.vec_bytes);
+              aux_project(*q.op(), q.reg[0].ptr, 0.);
+            }
+          catch (...)
+            {
+              // leave no half-built request behind; the caller sees the
+              // error of this call and the request's FAILED state
+              for (auto &r : q.reg)
+                r.release();
+              if (q.handle)
+                dg_destroy(q.handle);
+              q.handle = nullptr;
+              q.state  = DG_REQUEST_FAILED;
+              throw;
+            }
           q.state = DG_REQUEST_ACTIVE;
           used += need;
           active.push_back(q.id);

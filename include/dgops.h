@@ -338,7 +338,9 @@ enum dg_request_state
   DG_REQUEST_QUEUED  = 0,
   DG_REQUEST_ACTIVE  = 1,
   DG_REQUEST_DONE    = 2,
-  DG_REQUEST_UNKNOWN = 3
+  DG_REQUEST_UNKNOWN = 3,
+  DG_REQUEST_FAILED  = 4  /* its operator or vectors could not be set up;
+                             the step call that admitted it reported why */
 };
 
 typedef struct dg_server dg_server;
