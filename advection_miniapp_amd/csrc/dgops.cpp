@@ -358,6 +358,13 @@ namespace dgops
       require(c.refine >= 0 && c.refine <= 12, "refine out of range");
       for (int d = 0; d < c.dim; ++d)
         require(c.n_coarse[d] >= 1 && c.hi[d] > c.lo[d], "bad mesh box");
+      // cell and neighbour indices are 32-bit
+      double cells = std::ldexp(1.0, c.dim * c.refine);
+      for (int d = 0; d < c.dim; ++d)
+        cells *= c.n_coarse[d];
+      require(cells <= 2147483647.,
+              "mesh too large: at most 2^31 - 1 cells per operator",
+              DG_ERR_UNSUPPORTED);
       require(c.perturbation >= 0 && c.perturbation < 0.5,
               "perturbation must be in [0, 0.5)");
       require(c.time_factor == DG_TIME_FACTOR_ONE || c.final_time > 0,

@@ -37,6 +37,11 @@ def test_config_default_and_validation_without_gpu():
     assert lib.dg_create(ctypes.byref(cfg), None, ctypes.byref(h)) == \
         _lib.DG_ERR_UNSUPPORTED
     assert b"degree" in lib.dg_last_error()
+    cfg.degree = 4
+    cfg.refine = 11  # 2^33 cells: beyond the 32-bit cell index
+    assert lib.dg_create(ctypes.byref(cfg), None, ctypes.byref(h)) == \
+        _lib.DG_ERR_UNSUPPORTED
+    assert b"too large" in lib.dg_last_error()
     assert lib.dg_lsrk_n_stages(_lib.DG_LSRK_5_4) == 5
     assert lib.dg_lsrk_n_stages(_lib.DG_LSRK_9_5) == 9
     assert lib.dg_lsrk_n_stages(17) == -1
