@@ -1,6 +1,6 @@
 """End-to-end runs of the reference's explicit driver loop
 (AdvectionProblem::run, advection_solver.cc:1220-1310) on the GPU against
-the same loop on the oracle: BASELINE configuration 1 (2D, degree 3,
+the same loop on the oracle: the 2D benchmark configuration (degree 3,
 32 x 32 cells on [-1, 1]^2, velocity (1, 0.15), u0 = sin(pi x) sin(pi y),
 LSRK(5,4) to t = 2 at Courant number 0.4) and its L2 error against the
 translated exact solution."""
@@ -52,8 +52,8 @@ def test_config1_l2_error_matches_oracle():
         o.lsrk_time_step(1, t, dt, v, r1, r2)
         t += dt
     err_o, nrm_o = o.l2_error(v, 2.0)
-    # the north star asks for 3 significant digits; the two runs agree to
-    # round-off over 2,105 operator applications
+    # the error is reported to 3 significant digits; the two runs agree
+    # to round-off over 2,105 operator applications
     assert abs(err - err_o) < 1e-9 * err_o, (err, err_o)
     assert abs(nrm - nrm_o) < 1e-12 * nrm_o
     assert rel_diff(u.cpu().numpy(), v) < 1e-10
