@@ -469,8 +469,8 @@ def main():
                          "HBM counter calibration of rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box of this build has one "
-              "MI355X; multi-GPU sharding is described, not run",
+        print("bench.py measures one GPU (--gpus 1): the library runs one "
+              "operator per GPU and has no multi-GPU decomposition",
               file=sys.stderr)
         sys.exit(2)
 
