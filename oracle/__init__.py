@@ -3,8 +3,7 @@
 A plain C++ restatement of the reference's algorithms (oracle.cpp, built
 into liboracle.so by oracle/Makefile). Only tests/, __graft_entry__.smoke()
 and bench.py's CPU-baseline leg use it, as the reference of the parity
-checks; the product never
-links or calls it.
+checks; the product never links or calls it.
 """
 import ctypes
 import os
