@@ -70,6 +70,7 @@ class DgSolverControl(ctypes.Structure):
         ("flexible", ctypes.c_int),
         ("prec_precision", ctypes.c_int),
         ("inner_iterations", ctypes.c_int),
+        ("cell_solver", ctypes.c_int),
         ("iterations", ctypes.c_int),
         ("initial_residual", ctypes.c_double),
         ("final_residual", ctypes.c_double),
@@ -124,6 +125,8 @@ SIGNATURES = {
     "dg_irk_vmult": (_i, [_vp, _vp, _vp]),
     "dg_irk_update_solution": (_i, [_vp, _vp, _vp]),
     "dg_irk_precondition_block_jacobi": (_i, [_vp, _vp, _vp]),
+    "dg_irk_precondition_cell_solver": (_i, [_vp, _vp, _vp]),
+    "dg_irk_cell_operator": (_i, [_vp, _vp, _vp]),
     "dg_irk_coefficients": (_i, [_i, _dp, _dp, _dp]),
     "dg_irk_solve": (_i, [_vp, _vp, _vp, ctypes.POINTER(DgSolverControl)]),
     "dg_irk_time_step": (_i, [_vp, _d, _d, _vp,

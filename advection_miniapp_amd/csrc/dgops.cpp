@@ -17,6 +17,20 @@ namespace dgops
     return m;
   }
 
+  std::map<uint32_t, ToCollocLaunch> &
+  colloc_registry()
+  {
+    static std::map<uint32_t, ToCollocLaunch> m;
+    return m;
+  }
+
+  std::map<uint32_t, CellGmresLaunch> &
+  cell_gmres_registry()
+  {
+    static std::map<uint32_t, CellGmresLaunch> m;
+    return m;
+  }
+
   std::map<uint32_t, MassLaunch> &
   mass_registry()
   {

@@ -13,7 +13,7 @@ OUT = os.path.join(HERE, "generated")
 # reads the precomputed geometry tables, geo 1 evaluates axis-aligned
 # geometry and the speed per point (SIPG with a variable speed), geo 2 is
 # axis-aligned with a constant speed (products of 1D weights)
-FIN_INVMASS, FIN_INTEGRATE = 0, 1
+FIN_INVMASS, FIN_INTEGRATE, FIN_LOCAL = 0, 1, 2
 EPI_STORE, EPI_LSRK = 0, 1
 GEO_TABLE, GEO_ANALYTIC, GEO_UNIFORM = 0, 1, 2
 ADVECTION_GEO = (GEO_TABLE, GEO_UNIFORM)
@@ -38,6 +38,11 @@ def op_variants():
     for dim, n, nc, fp32, cart in itertools.product(
             (2, 3), range(2, 8), (2, 3, 4), (0, 1), ADVECTION_GEO):
         v.append((dim, n, nc, fp32, cart, 1, 0, FIN_INTEGRATE, EPI_STORE))
+    # cell-local stage operator of the batched cell solver (collocation
+    # basis), 1..4 stages
+    for dim, n, nc, fp32, cart in itertools.product(
+            (2, 3), range(2, 8), (1, 2, 3, 4), (0, 1), ADVECTION_GEO):
+        v.append((dim, n, nc, fp32, cart, 1, 0, FIN_LOCAL, EPI_STORE))
     # advection + SIPG diffusion, axis-aligned: explicit (apply, LSRK stage)
     # and implicit operator, fp64 and fp32
     for dim, n, fp32, geo in itertools.product((2, 3), range(2, 9), (0, 1),
@@ -74,6 +79,16 @@ def mass_variants():
             itertools.product((2, 3), range(2, 9), (0, 1))]
 
 
+def colloc_variants():
+    # batched cell solver: collocation transform and the per-cell update
+    v = []
+    for dim, n, fp32 in itertools.product((2, 3), range(2, 8), (0, 1)):
+        v.append(("colloc", dim, n, fp32))
+        for ns in (1, 2, 3, 4):
+            v.append(("cellgmres", dim, n, ns, fp32))
+    return v
+
+
 def batch_variants():
     # serving loop: batched explicit LSRK stages, 3D, degrees 1..7
     return [("batch", 3, n) for n in range(2, 9)]
@@ -91,6 +106,20 @@ def emit(path, variants):
             lines.append(
                 f"  static BatchRegistrar s{i}(batch_key({dim}, {n}), "
                 f"&launch_batch<{dim}, {n}>);")
+            continue
+        if var[0] == "colloc":
+            _, dim, n, fp32 = var
+            T = "float" if fp32 else "double"
+            lines.append(
+                f"  static CollocRegistrar c{i}(colloc_key({dim}, {n}, {fp32}), "
+                f"&launch_to_colloc<{dim}, {n}, {T}>);")
+            continue
+        if var[0] == "cellgmres":
+            _, dim, n, ns, fp32 = var
+            T = "float" if fp32 else "double"
+            lines.append(
+                f"  static CellGmresRegistrar q{i}(cell_gmres_key({dim}, {n}, "
+                f"{ns}, {fp32}), &launch_cell_gmres<{dim}, {n}, {ns}, {T}>);")
             continue
         if var[0] == "mass":
             _, dim, n, fp32 = var
@@ -134,11 +163,11 @@ def main(n_files=24):
             return v[2] ** (v[1] + 1) * v[3]
         if v[0] == "burgers":
             return v[3] ** (v[1] + 1)
-        if v[0] in ("batch", "mass"):
+        if v[0] in ("batch", "mass", "colloc", "cellgmres"):
             return v[2] ** (v[1] + 1)
         return v[1] ** (v[0] + 1) * v[2] * (2 if v[6] else 1)
     variants = (op_variants() + fdm_variants() + burgers_variants() +
-                batch_variants() + mass_variants())
+                batch_variants() + mass_variants() + colloc_variants())
     # spread heavy (3D, large n) variants evenly
     variants.sort(key=lambda v: -weight(v))
     groups = [[] for _ in range(n_files)]

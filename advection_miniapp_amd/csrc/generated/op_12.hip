@@ -3,32 +3,45 @@
 namespace dgops
 {
   static OpRegistrar r0(op_key(3, 7, 4, 1, 0, 1, 0, 1, 0), &launch_op<3, 7, 4, float, 0, true, false, 1, 0>);
-  static FdmRegistrar f1(fdm_key(3, 7, 3, 0), &launch_fdm<3, 7, 3, double, double, double>);
-  static OpRegistrar r2(op_key(3, 7, 1, 1, 2, 0, 1, 0, 1), &launch_op<3, 7, 1, float, 2, false, true, 0, 1>);
-  static OpRegistrar r3(op_key(3, 6, 3, 1, 2, 1, 0, 1, 0), &launch_op<3, 6, 3, float, 2, true, false, 1, 0>);
-  static OpRegistrar r4(op_key(3, 5, 4, 0, 2, 1, 0, 1, 0), &launch_op<3, 5, 4, double, 2, true, false, 1, 0>);
-  static FdmRegistrar f5(fdm_key(2, 8, 4, 2), &launch_fdm<2, 8, 4, float, double, double>);
-  static OpRegistrar r6(op_key(3, 6, 1, 0, 0, 1, 0, 1, 0), &launch_op<3, 6, 1, double, 0, true, false, 1, 0>);
-  static OpRegistrar r7(op_key(3, 5, 1, 1, 1, 0, 1, 0, 0), &launch_op<3, 5, 1, float, 1, false, true, 0, 0>);
-  static OpRegistrar r8(op_key(2, 8, 1, 0, 2, 0, 1, 0, 1), &launch_op<2, 8, 1, double, 2, false, true, 0, 1>);
-  static FdmRegistrar f9(fdm_key(2, 6, 4, 2), &launch_fdm<2, 6, 4, float, double, double>);
-  static FdmRegistrar f10(fdm_key(2, 7, 2, 0), &launch_fdm<2, 7, 2, double, double, double>);
-  static MassRegistrar m11(mass_key(3, 5, 0), &launch_mass<3, 5, double>);
-  static OpRegistrar r12(op_key(3, 4, 1, 1, 1, 1, 1, 1, 0), &launch_op<3, 4, 1, float, 1, true, true, 1, 0>);
-  static OpRegistrar r13(op_key(2, 6, 1, 0, 1, 0, 1, 0, 1), &launch_op<2, 6, 1, double, 1, false, true, 0, 1>);
-  static OpRegistrar r14(op_key(2, 7, 1, 0, 2, 0, 0, 0, 1), &launch_op<2, 7, 1, double, 2, false, false, 0, 1>);
-  static OpRegistrar r15(op_key(3, 4, 1, 1, 2, 0, 0, 0, 0), &launch_op<3, 4, 1, float, 2, false, false, 0, 0>);
-  static OpRegistrar r16(op_key(2, 5, 1, 0, 1, 1, 1, 1, 0), &launch_op<2, 5, 1, double, 1, true, true, 1, 0>);
-  static OpRegistrar r17(op_key(2, 6, 1, 1, 0, 0, 0, 0, 0), &launch_op<2, 6, 1, float, 0, false, false, 0, 0>);
-  static OpRegistrar r18(op_key(3, 3, 2, 1, 0, 1, 0, 1, 0), &launch_op<3, 3, 2, float, 0, true, false, 1, 0>);
-  static OpRegistrar r19(op_key(2, 4, 1, 0, 2, 0, 1, 0, 0), &launch_op<2, 4, 1, double, 2, false, true, 0, 0>);
-  static FdmRegistrar f20(fdm_key(2, 5, 1, 2), &launch_fdm<2, 5, 1, float, double, double>);
-  static FdmRegistrar f21(fdm_key(2, 3, 3, 0), &launch_fdm<2, 3, 3, double, double, double>);
-  static OpRegistrar r22(op_key(3, 2, 4, 0, 2, 1, 0, 1, 0), &launch_op<3, 2, 4, double, 2, true, false, 1, 0>);
-  static OpRegistrar r23(op_key(2, 3, 1, 1, 2, 0, 1, 0, 0), &launch_op<2, 3, 1, float, 2, false, true, 0, 0>);
-  static OpRegistrar r24(op_key(3, 2, 1, 0, 2, 0, 1, 0, 0), &launch_op<3, 2, 1, double, 2, false, true, 0, 0>);
-  static OpRegistrar r25(op_key(2, 3, 1, 1, 2, 1, 0, 1, 0), &launch_op<2, 3, 1, float, 2, true, false, 1, 0>);
-  static OpRegistrar r26(op_key(3, 2, 1, 1, 0, 1, 0, 1, 0), &launch_op<3, 2, 1, float, 0, true, false, 1, 0>);
-  static FdmRegistrar f27(fdm_key(3, 2, 1, 2), &launch_fdm<3, 2, 1, float, double, double>);
-  static BurgersRegistrar g28(burgers_key(3, 2, 0, 1), &launch_burgers<3, 2, 1, false, 1>);
+  static OpRegistrar r1(op_key(3, 7, 3, 0, 0, 1, 0, 1, 0), &launch_op<3, 7, 3, double, 0, true, false, 1, 0>);
+  static OpRegistrar r2(op_key(3, 7, 2, 1, 0, 1, 0, 1, 0), &launch_op<3, 7, 2, float, 0, true, false, 1, 0>);
+  static OpRegistrar r3(op_key(3, 8, 1, 0, 2, 0, 0, 0, 1), &launch_op<3, 8, 1, double, 2, false, false, 0, 1>);
+  static OpRegistrar r4(op_key(3, 6, 2, 0, 0, 1, 0, 1, 0), &launch_op<3, 6, 2, double, 0, true, false, 1, 0>);
+  static OpRegistrar r5(op_key(3, 5, 4, 0, 2, 1, 0, 1, 0), &launch_op<3, 5, 4, double, 2, true, false, 1, 0>);
+  static FdmRegistrar f6(fdm_key(3, 7, 1, 0), &launch_fdm<3, 7, 1, double, double, double>);
+  static OpRegistrar r7(op_key(3, 5, 3, 0, 2, 1, 0, 2, 0), &launch_op<3, 5, 3, double, 2, true, false, 2, 0>);
+  static OpRegistrar r8(op_key(3, 6, 1, 1, 0, 0, 0, 0, 0), &launch_op<3, 6, 1, float, 0, false, false, 0, 0>);
+  static CellGmresRegistrar q9(cell_gmres_key(3, 6, 4, 0), &launch_cell_gmres<3, 6, 4, double>);
+  static OpRegistrar r10(op_key(3, 5, 1, 1, 2, 0, 1, 0, 1), &launch_op<3, 5, 1, float, 2, false, true, 0, 1>);
+  static OpRegistrar r11(op_key(2, 8, 1, 0, 1, 0, 1, 0, 0), &launch_op<2, 8, 1, double, 1, false, true, 0, 0>);
+  static OpRegistrar r12(op_key(2, 6, 4, 1, 0, 1, 0, 1, 0), &launch_op<2, 6, 4, float, 0, true, false, 1, 0>);
+  static OpRegistrar r13(op_key(2, 7, 2, 0, 0, 1, 0, 2, 0), &launch_op<2, 7, 2, double, 0, true, false, 2, 0>);
+  static OpRegistrar r14(op_key(2, 6, 3, 0, 2, 1, 0, 2, 0), &launch_op<2, 6, 3, double, 2, true, false, 2, 0>);
+  static MassRegistrar m15(mass_key(3, 5, 0), &launch_mass<3, 5, double>);
+  static OpRegistrar r16(op_key(3, 4, 2, 1, 0, 1, 0, 1, 0), &launch_op<3, 4, 2, float, 0, true, false, 1, 0>);
+  static MassRegistrar m17(mass_key(2, 8, 0), &launch_mass<2, 8, double>);
+  static OpRegistrar r18(op_key(2, 6, 1, 0, 2, 0, 1, 0, 0), &launch_op<2, 6, 1, double, 2, false, true, 0, 0>);
+  static OpRegistrar r19(op_key(2, 7, 1, 0, 0, 0, 0, 0, 1), &launch_op<2, 7, 1, double, 0, false, false, 0, 1>);
+  static CellGmresRegistrar q20(cell_gmres_key(2, 7, 1, 1), &launch_cell_gmres<2, 7, 1, float>);
+  static OpRegistrar r21(op_key(3, 4, 1, 1, 2, 1, 0, 1, 0), &launch_op<3, 4, 1, float, 2, true, false, 1, 0>);
+  static CellGmresRegistrar q22(cell_gmres_key(3, 4, 2, 0), &launch_cell_gmres<3, 4, 2, double>);
+  static OpRegistrar r23(op_key(2, 5, 1, 1, 1, 1, 1, 1, 0), &launch_op<2, 5, 1, float, 1, true, true, 1, 0>);
+  static OpRegistrar r24(op_key(2, 6, 1, 0, 0, 1, 0, 1, 0), &launch_op<2, 6, 1, double, 0, true, false, 1, 0>);
+  static CellGmresRegistrar q25(cell_gmres_key(2, 6, 2, 1), &launch_cell_gmres<2, 6, 2, float>);
+  static OpRegistrar r26(op_key(3, 3, 1, 0, 1, 1, 1, 1, 0), &launch_op<3, 3, 1, double, 1, true, true, 1, 0>);
+  static OpRegistrar r27(op_key(2, 4, 1, 0, 2, 0, 1, 0, 0), &launch_op<2, 4, 1, double, 2, false, true, 0, 0>);
+  static OpRegistrar r28(op_key(2, 5, 1, 1, 0, 1, 0, 2, 0), &launch_op<2, 5, 1, float, 0, true, false, 2, 0>);
+  static OpRegistrar r29(op_key(2, 3, 4, 1, 2, 1, 0, 2, 0), &launch_op<2, 3, 4, float, 2, true, false, 2, 0>);
+  static OpRegistrar r30(op_key(3, 3, 1, 1, 0, 1, 0, 2, 0), &launch_op<3, 3, 1, float, 0, true, false, 2, 0>);
+  static CellGmresRegistrar q31(cell_gmres_key(3, 3, 4, 1), &launch_cell_gmres<3, 3, 4, float>);
+  static FdmRegistrar f32(fdm_key(2, 4, 1, 1), &launch_fdm<2, 4, 1, float, float, float>);
+  static OpRegistrar r33(op_key(2, 3, 2, 1, 2, 1, 0, 2, 0), &launch_op<2, 3, 2, float, 2, true, false, 2, 0>);
+  static FdmRegistrar f34(fdm_key(3, 2, 3, 0), &launch_fdm<3, 2, 3, double, double, double>);
+  static OpRegistrar r35(op_key(3, 2, 1, 0, 2, 1, 1, 1, 0), &launch_op<3, 2, 1, double, 2, true, true, 1, 0>);
+  static OpRegistrar r36(op_key(2, 3, 1, 0, 2, 1, 0, 2, 0), &launch_op<2, 3, 1, double, 2, true, false, 2, 0>);
+  static OpRegistrar r37(op_key(2, 2, 3, 1, 0, 1, 0, 1, 0), &launch_op<2, 2, 3, float, 0, true, false, 1, 0>);
+  static OpRegistrar r38(op_key(2, 2, 2, 0, 2, 1, 0, 2, 0), &launch_op<2, 2, 2, double, 2, true, false, 2, 0>);
+  static FdmRegistrar f39(fdm_key(3, 2, 1, 2), &launch_fdm<3, 2, 1, float, double, double>);
+  static OpRegistrar r40(op_key(2, 2, 1, 0, 0, 1, 0, 2, 0), &launch_op<2, 2, 1, double, 0, true, false, 2, 0>);
+  static BurgersRegistrar g41(burgers_key(3, 2, 0, 1), &launch_burgers<3, 2, 1, false, 1>);
 } // namespace dgops

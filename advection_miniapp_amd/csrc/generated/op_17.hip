@@ -2,32 +2,45 @@
 
 namespace dgops
 {
-  static OpRegistrar r0(op_key(3, 8, 1, 0, 1, 0, 1, 0, 0), &launch_op<3, 8, 1, double, 1, false, true, 0, 0>);
-  static OpRegistrar r1(op_key(3, 6, 4, 1, 0, 1, 0, 1, 0), &launch_op<3, 6, 4, float, 0, true, false, 1, 0>);
-  static OpRegistrar r2(op_key(3, 8, 1, 0, 0, 0, 0, 0, 0), &launch_op<3, 8, 1, double, 0, false, false, 0, 0>);
-  static OpRegistrar r3(op_key(3, 6, 2, 0, 2, 1, 0, 1, 0), &launch_op<3, 6, 2, double, 2, true, false, 1, 0>);
-  static FdmRegistrar f4(fdm_key(3, 5, 4, 2), &launch_fdm<3, 5, 4, float, double, double>);
-  static FdmRegistrar f5(fdm_key(3, 5, 3, 0), &launch_fdm<3, 5, 3, double, double, double>);
-  static FdmRegistrar f6(fdm_key(3, 6, 1, 1), &launch_fdm<3, 6, 1, float, float, float>);
-  static OpRegistrar r7(op_key(3, 5, 1, 1, 2, 1, 1, 1, 0), &launch_op<3, 5, 1, float, 2, true, true, 1, 0>);
-  static OpRegistrar r8(op_key(2, 8, 1, 1, 2, 0, 1, 0, 0), &launch_op<2, 8, 1, float, 2, false, true, 0, 0>);
-  static FdmRegistrar f9(fdm_key(3, 4, 3, 0), &launch_fdm<3, 4, 3, double, double, double>);
-  static OpRegistrar r10(op_key(2, 6, 3, 1, 0, 1, 0, 1, 0), &launch_op<2, 6, 3, float, 0, true, false, 1, 0>);
-  static OpRegistrar r11(op_key(2, 8, 1, 0, 2, 0, 0, 0, 1), &launch_op<2, 8, 1, double, 2, false, false, 0, 1>);
-  static FdmRegistrar f12(fdm_key(2, 8, 1, 1), &launch_fdm<2, 8, 1, float, float, float>);
-  static OpRegistrar r13(op_key(2, 6, 1, 1, 1, 0, 1, 0, 0), &launch_op<2, 6, 1, float, 1, false, true, 0, 0>);
-  static OpRegistrar r14(op_key(2, 7, 1, 0, 2, 1, 0, 1, 0), &launch_op<2, 7, 1, double, 2, true, false, 1, 0>);
-  static OpRegistrar r15(op_key(2, 4, 4, 0, 0, 1, 0, 1, 0), &launch_op<2, 4, 4, double, 0, true, false, 1, 0>);
-  static OpRegistrar r16(op_key(2, 5, 1, 1, 1, 0, 1, 0, 1), &launch_op<2, 5, 1, float, 1, false, true, 0, 1>);
-  static OpRegistrar r17(op_key(2, 6, 1, 1, 2, 1, 0, 1, 0), &launch_op<2, 6, 1, float, 2, true, false, 1, 0>);
-  static OpRegistrar r18(op_key(3, 3, 1, 0, 2, 0, 1, 0, 0), &launch_op<3, 3, 1, double, 2, false, true, 0, 0>);
-  static OpRegistrar r19(op_key(2, 4, 1, 1, 1, 1, 1, 1, 0), &launch_op<2, 4, 1, float, 1, true, true, 1, 0>);
-  static OpRegistrar r20(op_key(2, 3, 4, 1, 0, 1, 0, 1, 0), &launch_op<2, 3, 4, float, 0, true, false, 1, 0>);
-  static FdmRegistrar f21(fdm_key(3, 3, 1, 2), &launch_fdm<3, 3, 1, float, double, double>);
-  static FdmRegistrar f22(fdm_key(2, 4, 1, 2), &launch_fdm<2, 4, 1, float, double, double>);
-  static FdmRegistrar f23(fdm_key(2, 3, 2, 2), &launch_fdm<2, 3, 2, float, double, double>);
-  static OpRegistrar r24(op_key(3, 2, 1, 1, 1, 1, 1, 1, 0), &launch_op<3, 2, 1, float, 1, true, true, 1, 0>);
-  static BurgersRegistrar g25(burgers_key(2, 3, 0, 1), &launch_burgers<2, 3, 3, false, 1>);
-  static OpRegistrar r26(op_key(2, 2, 2, 1, 0, 1, 0, 1, 0), &launch_op<2, 2, 2, float, 0, true, false, 1, 0>);
-  static OpRegistrar r27(op_key(2, 2, 1, 0, 0, 0, 0, 0, 1), &launch_op<2, 2, 1, double, 0, false, false, 0, 1>);
+  static OpRegistrar r0(op_key(3, 7, 4, 1, 2, 1, 0, 2, 0), &launch_op<3, 7, 4, float, 2, true, false, 2, 0>);
+  static OpRegistrar r1(op_key(3, 7, 3, 0, 2, 1, 0, 2, 0), &launch_op<3, 7, 3, double, 2, true, false, 2, 0>);
+  static OpRegistrar r2(op_key(3, 7, 2, 1, 2, 1, 0, 2, 0), &launch_op<3, 7, 2, float, 2, true, false, 2, 0>);
+  static OpRegistrar r3(op_key(3, 8, 1, 0, 2, 1, 0, 1, 0), &launch_op<3, 8, 1, double, 2, true, false, 1, 0>);
+  static OpRegistrar r4(op_key(3, 6, 2, 0, 2, 1, 0, 2, 0), &launch_op<3, 6, 2, double, 2, true, false, 2, 0>);
+  static OpRegistrar r5(op_key(3, 5, 4, 1, 0, 1, 0, 2, 0), &launch_op<3, 5, 4, float, 0, true, false, 2, 0>);
+  static MassRegistrar m6(mass_key(3, 7, 1), &launch_mass<3, 7, float>);
+  static FdmRegistrar f7(fdm_key(3, 5, 3, 2), &launch_fdm<3, 5, 3, float, double, double>);
+  static OpRegistrar r8(op_key(3, 6, 1, 1, 2, 1, 0, 1, 0), &launch_op<3, 6, 1, float, 2, true, false, 1, 0>);
+  static CellGmresRegistrar q9(cell_gmres_key(3, 6, 4, 1), &launch_cell_gmres<3, 6, 4, float>);
+  static OpRegistrar r10(op_key(2, 7, 3, 0, 0, 1, 0, 1, 0), &launch_op<2, 7, 3, double, 0, true, false, 1, 0>);
+  static OpRegistrar r11(op_key(2, 8, 1, 0, 2, 1, 1, 1, 0), &launch_op<2, 8, 1, double, 2, true, true, 1, 0>);
+  static OpRegistrar r12(op_key(2, 6, 4, 1, 2, 1, 0, 2, 0), &launch_op<2, 6, 4, float, 2, true, false, 2, 0>);
+  static OpRegistrar r13(op_key(2, 7, 1, 0, 1, 0, 1, 0, 1), &launch_op<2, 7, 1, double, 1, false, true, 0, 1>);
+  static FdmRegistrar f14(fdm_key(2, 6, 3, 2), &launch_fdm<2, 6, 3, float, double, double>);
+  static CellGmresRegistrar q15(cell_gmres_key(3, 5, 3, 0), &launch_cell_gmres<3, 5, 3, double>);
+  static OpRegistrar r16(op_key(3, 4, 2, 1, 2, 1, 0, 2, 0), &launch_op<3, 4, 2, float, 2, true, false, 2, 0>);
+  static OpRegistrar r17(op_key(2, 5, 4, 1, 2, 1, 0, 1, 0), &launch_op<2, 5, 4, float, 2, true, false, 1, 0>);
+  static OpRegistrar r18(op_key(2, 6, 1, 1, 1, 1, 1, 1, 0), &launch_op<2, 6, 1, float, 1, true, true, 1, 0>);
+  static OpRegistrar r19(op_key(2, 7, 1, 0, 0, 1, 0, 1, 0), &launch_op<2, 7, 1, double, 0, true, false, 1, 0>);
+  static OpRegistrar r20(op_key(3, 3, 4, 0, 2, 1, 0, 1, 0), &launch_op<3, 3, 4, double, 2, true, false, 1, 0>);
+  static OpRegistrar r21(op_key(2, 4, 4, 0, 0, 1, 0, 2, 0), &launch_op<2, 4, 4, double, 0, true, false, 2, 0>);
+  static CellGmresRegistrar q22(cell_gmres_key(3, 4, 2, 1), &launch_cell_gmres<3, 4, 2, float>);
+  static FdmRegistrar f23(fdm_key(2, 5, 2, 1), &launch_fdm<2, 5, 2, float, float, float>);
+  static OpRegistrar r24(op_key(2, 6, 1, 0, 2, 1, 0, 2, 0), &launch_op<2, 6, 1, double, 2, true, false, 2, 0>);
+  static OpRegistrar r25(op_key(2, 4, 3, 1, 0, 1, 0, 1, 0), &launch_op<2, 4, 3, float, 0, true, false, 1, 0>);
+  static OpRegistrar r26(op_key(3, 3, 1, 1, 1, 0, 1, 0, 1), &launch_op<3, 3, 1, float, 1, false, true, 0, 1>);
+  static OpRegistrar r27(op_key(2, 4, 1, 1, 1, 1, 1, 1, 0), &launch_op<2, 4, 1, float, 1, true, true, 1, 0>);
+  static MassRegistrar m28(mass_key(2, 5, 0), &launch_mass<2, 5, double>);
+  static OpRegistrar r29(op_key(3, 3, 1, 0, 0, 0, 0, 0, 1), &launch_op<3, 3, 1, double, 0, false, false, 0, 1>);
+  static FdmRegistrar f30(fdm_key(3, 3, 1, 0), &launch_fdm<3, 3, 1, double, double, double>);
+  static OpRegistrar r31(op_key(2, 4, 1, 1, 0, 0, 0, 0, 0), &launch_op<2, 4, 1, float, 0, false, false, 0, 0>);
+  static MassRegistrar m32(mass_key(2, 4, 0), &launch_mass<2, 4, double>);
+  static OpRegistrar r33(op_key(2, 3, 1, 0, 2, 0, 1, 0, 1), &launch_op<2, 3, 1, double, 2, false, true, 0, 1>);
+  static OpRegistrar r34(op_key(2, 2, 4, 1, 0, 1, 0, 1, 0), &launch_op<2, 2, 4, float, 0, true, false, 1, 0>);
+  static OpRegistrar r35(op_key(3, 2, 1, 1, 2, 0, 1, 0, 1), &launch_op<3, 2, 1, float, 2, false, true, 0, 1>);
+  static FdmRegistrar f36(fdm_key(2, 3, 1, 2), &launch_fdm<2, 3, 1, float, double, double>);
+  static OpRegistrar r37(op_key(2, 2, 3, 1, 2, 1, 0, 2, 0), &launch_op<2, 2, 3, float, 2, true, false, 2, 0>);
+  static OpRegistrar r38(op_key(3, 2, 1, 1, 0, 1, 0, 2, 0), &launch_op<3, 2, 1, float, 0, true, false, 2, 0>);
+  static CellGmresRegistrar q39(cell_gmres_key(3, 2, 1, 0), &launch_cell_gmres<3, 2, 1, double>);
+  static FdmRegistrar f40(fdm_key(2, 2, 1, 1), &launch_fdm<2, 2, 1, float, float, float>);
 } // namespace dgops

@@ -202,50 +202,81 @@ namespace dgops
     {
       if (op.d_fdm_d.ptr)
         return;
-      const int      n = op.n;
-      const Basis1D &B = op.basis;
-      std::vector<double> K(n * n, 0.), M(n * n, 0.);
-      for (int q = 0; q < n; ++q)
-        for (int i = 0; i < n; ++i)
-          for (int j = 0; j < n; ++j)
-            {
-              K[i * n + j] -= B.G[q][i] * B.S[q][j] * B.gauss_wts[q];
-              M[i * n + j] += B.S[q][i] * B.S[q][j] * B.gauss_wts[q];
-            }
-      const double alpha = op.cfg.flux_alpha;
-      // nodal basis at 0 and 1: phi_i(0) = delta_i0, phi_i(1) = delta_i,n-1
-      K[0] += -(0.5 - alpha * 0.5);
-      K[(n - 1) * n + (n - 1)] += (0.5 + alpha * 0.5);
-      const auto Minv = invert_dense(M, n);
-      std::vector<double> D(n * n, 0.);
-      for (int i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j)
-          for (int k = 0; k < n; ++k)
-            D[i * n + j] += Minv[i * n + k] * K[k * n + j];
-      const RealPairEig   E = real_pair_eigen(D, n);
-      std::vector<double> VinvMinv(n * n, 0.);
-      for (int i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j)
-          for (int k = 0; k < n; ++k)
-            VinvMinv[i * n + j] += E.Vinv[i * n + k] * Minv[k * n + j];
-      FdmDev<double> fd1 = {}, fdS = {};
-      FdmDev<float>  ff1 = {}, ffS = {};
-      fill_spatial(fd1, E, VinvMinv, n);
-      fill_spatial(ff1, E, VinvMinv, n);
-      fill_stages(fd1, nullptr);
-      fill_stages(ff1, nullptr);
-      fdS = fd1;
-      ffS = ff1;
+      const int      n     = op.n;
+      const Basis1D &B     = op.basis;
+      const double   alpha = op.cfg.flux_alpha;
+      // tables [basis][stages]: basis 0 = Gauss-Lobatto nodal (the dof
+      // basis), 1 = Lagrange at the Gauss points (the collocation basis of
+      // the batched cell solver, advection_solver_irk_real.cc:1492-1494);
+      // stages 0 = single stage, 1 = the IRK stage transform
+      FdmDev<double> hd[4] = {};
+      FdmDev<float>  hf[4] = {};
       if (op.cfg.irk_stages >= 1 && op.cfg.irk_stages <= 4)
+        setup_irk(op);
+      for (int colloc = 0; colloc < 2; ++colloc)
         {
-          setup_irk(op);
-          fill_stages(fdS, &op.irk);
-          fill_stages(ffS, &op.irk);
+          // K(i,j) = -int phi_i' phi_j + upwind end terms, M(i,j) = int
+          // phi_i phi_j, both with QGauss(n) (:1497-1522)
+          std::vector<double> K(n * n, 0.), M(n * n, 0.);
+          if (!colloc)
+            {
+              for (int q = 0; q < n; ++q)
+                for (int i = 0; i < n; ++i)
+                  for (int j = 0; j < n; ++j)
+                    {
+                      K[i * n + j] -= B.G[q][i] * B.S[q][j] * B.gauss_wts[q];
+                      M[i * n + j] += B.S[q][i] * B.S[q][j] * B.gauss_wts[q];
+                    }
+              // nodal basis at 0 and 1: phi_i(0) = delta_i0, phi_i(1) =
+              // delta_i,n-1
+              K[0] += -(0.5 - alpha * 0.5);
+              K[(n - 1) * n + (n - 1)] += (0.5 + alpha * 0.5);
+            }
+          else
+            {
+              // phi_i(x_q) = delta_iq, phi_i'(x_q) = D[q][i]; at the ends
+              // the Gauss-Lagrange values l0, l1
+              for (int i = 0; i < n; ++i)
+                {
+                  M[i * n + i] = B.gauss_wts[i];
+                  for (int j = 0; j < n; ++j)
+                    K[i * n + j] = -B.D[j][i] * B.gauss_wts[j] -
+                                   B.l0[i] * B.l0[j] * (0.5 - alpha * 0.5) +
+                                   B.l1[i] * B.l1[j] * (0.5 + alpha * 0.5);
+                }
+            }
+          const auto Minv = invert_dense(M, n);
+          std::vector<double> D(n * n, 0.);
+          for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j)
+              for (int k = 0; k < n; ++k)
+                D[i * n + j] += Minv[i * n + k] * K[k * n + j];
+          const RealPairEig   E = real_pair_eigen(D, n);
+          std::vector<double> VinvMinv(n * n, 0.);
+          for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j)
+              for (int k = 0; k < n; ++k)
+                VinvMinv[i * n + j] += E.Vinv[i * n + k] * Minv[k * n + j];
+          FdmDev<double> fd1 = {};
+          FdmDev<float>  ff1 = {};
+          fill_spatial(fd1, E, VinvMinv, n);
+          fill_spatial(ff1, E, VinvMinv, n);
+          fill_stages(fd1, nullptr);
+          fill_stages(ff1, nullptr);
+          FdmDev<double> fdS = fd1;
+          FdmDev<float>  ffS = ff1;
+          if (op.cfg.irk_stages >= 1 && op.cfg.irk_stages <= 4)
+            {
+              fill_stages(fdS, &op.irk);
+              fill_stages(ffS, &op.irk);
+            }
+          hd[2 * colloc]     = fd1;
+          hd[2 * colloc + 1] = fdS;
+          hf[2 * colloc]     = ff1;
+          hf[2 * colloc + 1] = ffS;
         }
-      op.d_fdm_d.alloc(2 * sizeof(FdmDev<double>));
-      op.d_fdm_f.alloc(2 * sizeof(FdmDev<float>));
-      FdmDev<double> hd[2] = {fd1, fdS};
-      FdmDev<float>  hf[2] = {ff1, ffS};
+      op.d_fdm_d.alloc(sizeof(hd));
+      op.d_fdm_f.alloc(sizeof(hf));
       hip_check(hipMemcpy(op.d_fdm_d.ptr, hd, sizeof(hd), hipMemcpyHostToDevice),
                 "upload fdm");
       hip_check(hipMemcpy(op.d_fdm_f.ptr, hf, sizeof(hf), hipMemcpyHostToDevice),
@@ -259,7 +290,8 @@ namespace dgops
     // prec: 0 fp64, 1 fp32 vectors, 2 fp32 arithmetic on fp64 vectors
     void
     launch_fdm_variant(Operator &op, int ns, int prec, double vel_scale,
-                       double inv_dt, const void *src, void *dst)
+                       double inv_dt, const void *src, void *dst,
+                       bool colloc = false)
     {
       setup_fdm(op);
       FdmParams P     = {};
@@ -280,7 +312,7 @@ namespace dgops
                           (const char *)op.d_fdm_f.ptr;
       const size_t sz = prec == 0 ? sizeof(FdmDev<double>) :
                                     sizeof(FdmDev<float>);
-      const void *tables = tab + (ns == 1 ? 0 : sz);
+      const void *tables = tab + (2 * (colloc ? 1 : 0) + (ns == 1 ? 0 : 1)) * sz;
       hip_check(it->second(P, tables, src, dst, op.stream), "fdm launch");
     }
 
@@ -356,6 +388,81 @@ namespace dgops
           launch_operator(op, P, s, false, true, fin_integrate, epi_store,
                           &io);
         }
+    }
+
+    // the cell-local stage operator in the Gauss-point collocation basis
+    // (CellwiseOperator::vmult, advection_solver_irk_real.cc:545-711): the
+    // stage-coupled operator of vmult_irk with the neighbour traces zero,
+    // values at the Gauss points in and out
+    void
+    cell_operator(Operator &op, bool fp32, double t, double dt,
+                  const void *src, void *dst)
+    {
+      setup_irk(op);
+      const int s = op.irk.s;
+      OpParams  P = op.base_params(t, s);
+      P.sign      = 1.;
+      P.has_mass  = 1;
+      for (int c = 0; c < s; ++c)
+        {
+          P.vel_factor[c] = velocity_time_factor(op.cfg, t + op.irk.c[c] * dt);
+          for (int b = 0; b < s; ++b)
+            P.mass[c][b] = op.irk.invA[c * s + b] / dt;
+        }
+      if (fp32)
+        {
+          OpBuffers<float> io = {};
+          io.src         = (const float *)src;
+          io.dst         = (float *)dst;
+          io.comp_stride = op.n_dofs;
+          io.scale       = 1.f;
+          launch_operator(op, P, s, true, true, fin_local, epi_store, &io);
+        }
+      else
+        {
+          OpBuffers<double> io = {};
+          io.src         = (const double *)src;
+          io.dst         = (double *)dst;
+          io.comp_stride = op.n_dofs;
+          io.scale       = 1.;
+          launch_operator(op, P, s, false, true, fin_local, epi_store, &io);
+        }
+    }
+
+    // The batched cell solver of the block-Jacobi preconditioner
+    // (do_batched_solver with one iteration, advection_solver_irk_real.cc:
+    // 2079-2126): the stage residual goes to the collocation basis, one
+    // right-preconditioned GMRES step per cell with the collocation FDM
+    // (x = alpha P^-1 r, alpha minimizing the cell residual), back to the
+    // nodal basis.
+    void
+    cell_gmres_precondition(Operator &op, bool fp32, double t, double dt,
+                            double vs, const void *src, void *dst)
+    {
+      setup_irk(op);
+      const int     s   = op.irk.s;
+      const int     dim = op.cfg.dim, n = op.n;
+      const int64_t N   = op.n_dofs * s;
+      const size_t  es  = fp32 ? 4 : 8;
+      void         *rq  = op.solver_vector(60, es * N);
+      void         *y   = op.solver_vector(61, es * N);
+      void         *z   = op.solver_vector(62, es * N);
+      const auto    tc  = colloc_registry().find(colloc_key(dim, n, fp32));
+      const auto    cg =
+        cell_gmres_registry().find(cell_gmres_key(dim, n, s, fp32));
+      require(tc != colloc_registry().end() &&
+                cg != cell_gmres_registry().end(),
+              "batched cell solver not built for this dim / degree / "
+              "number of stages",
+              DG_ERR_UNSUPPORTED);
+      hip_check(tc->second(op.mesh.n_cells, op.n_dofs, s,
+                           op.basis_ptr(fp32), src, rq, op.stream),
+                "collocation transform");
+      launch_fdm_variant(op, s, fp32 ? 1 : 0, vs, 1. / dt, rq, y, true);
+      cell_operator(op, fp32, t, dt, y, z);
+      hip_check(cg->second(op.mesh.n_cells, op.n_dofs, op.basis_ptr(fp32), rq,
+                           y, z, dst, op.stream),
+                "cell solver update");
     }
 
     double
@@ -676,6 +783,34 @@ dg_irk_precondition_block_jacobi(dg_operator *h, void *dst, const void *src)
 }
 
 int
+dg_irk_precondition_cell_solver(dg_operator *h, void *dst, const void *src)
+{
+  return guard([&] {
+    Operator *op = get(h);
+    require(dst && src, "null vector");
+    require(op->time_step != 0., "precondition: set_time with dt != 0",
+            DG_ERR_STATE);
+    // the stage count enters the averaged speed factor
+    setup_irk(*op);
+    cell_gmres_precondition(*op, op->fp32(), op->time, op->time_step,
+                            avg_stage_factor(*op, op->time, op->time_step),
+                            src, dst);
+  });
+}
+
+int
+dg_irk_cell_operator(dg_operator *h, void *dst, const void *src)
+{
+  return guard([&] {
+    Operator *op = get(h);
+    require(dst && src, "null vector");
+    require(op->time_step != 0., "cell operator: set_time with dt != 0",
+            DG_ERR_STATE);
+    cell_operator(*op, op->fp32(), op->time, op->time_step, src, dst);
+  });
+}
+
+int
 dg_irk_solve(dg_operator *h, void *stage, const void *rhs,
              dg_solver_control *ctl)
 {
@@ -714,7 +849,10 @@ dg_irk_solve(dg_operator *h, void *stage, const void *rhs,
             vmult_irk(*op, f32, t, dt, x, y);
           };
           in.P = [&, f32](const void *x, void *y) {
-            launch_fdm_variant(*op, s, f32 ? 1 : 0, vs, 1. / dt, x, y);
+            if (ctl->cell_solver)
+              cell_gmres_precondition(*op, f32, t, dt, vs, x, y);
+            else
+              launch_fdm_variant(*op, s, f32 ? 1 : 0, vs, 1. / dt, x, y);
           };
           vec_copy_convert(*op, N, DG_FP64, src, in.prec, bin);
           vec_axpby(*op, in.prec, N, 0., bin, 0., xin);
@@ -726,8 +864,15 @@ dg_irk_solve(dg_operator *h, void *stage, const void *rhs,
     else
       {
         const int prec = pp == DG_FP32 ? 2 : 0;
-        outer.P        = [&, prec](const void *x, void *y) {
-          launch_fdm_variant(*op, s, prec, vs, 1. / dt, x, y);
+        require(!ctl->cell_solver || prec == 0,
+                "the batched cell solver runs in the precision of its "
+                "vectors: FP32 through an inner solver (inner_iterations)",
+                DG_ERR_UNSUPPORTED);
+        outer.P = [&, prec](const void *x, void *y) {
+          if (ctl->cell_solver)
+            cell_gmres_precondition(*op, false, t, dt, vs, x, y);
+          else
+            launch_fdm_variant(*op, s, prec, vs, 1. / dt, x, y);
         };
       }
     double bn2;

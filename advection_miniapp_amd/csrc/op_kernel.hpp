@@ -387,7 +387,7 @@ namespace dgops
   // advection_solver_implicit_real.cc:1032-1058:
   //   F = 1/2 [(u- + u+) an + alpha |an| (u- - u+)] - skew an u-
   // with an = a . n dA on this cell's outward normal.
-  template <int dim, int n, int dir, typename T, bool diff>
+  template <int dim, int n, int dir, typename T, bool diff, bool local = false>
   __device__ __forceinline__ void
   face_terms(const OpParams &P,
              const DevBasis<double> *Bd,
@@ -415,7 +415,7 @@ namespace dgops
         for (int i = 1; i < n; ++i)
           um = fma(l[i], uq[i], um);
         T up = uplus[f];
-        if (active && nb[f] < 0)
+        if (!local && active && nb[f] < 0)
           // Dirichlet data, evaluated before the launch
           // (boundary_value_kernel)
           up = (T)P.bnd_value[(int64_t)P.bnd_index[cell * 2 * dim + f] *
@@ -493,7 +493,12 @@ namespace dgops
   enum Finalize : int
   {
     fin_inverse_mass = 0, // S^-1 diag(1/JxW): explicit right-hand side
-    fin_integrate    = 1  // S^T: operator action (implicit vmult)
+    fin_integrate    = 1, // S^T: operator action (implicit vmult)
+    fin_local        = 2  // cell-local operator in the Gauss-point
+                          // collocation basis: input and output at the
+                          // Gauss points, neighbour traces zero
+                          // (CellwiseOperator::vmult,
+                          // advection_solver_irk_real.cc:545-711)
   };
 
   enum Epilogue : int
@@ -534,6 +539,8 @@ namespace dgops
                 unsigned                              block)
   {
     static_assert(!diff || geo != geo_table, "SIPG needs axis-aligned cells");
+    static_assert(fin != fin_local || (!diff && epi == epi_store),
+                  "the cell-local operator is built for advection");
     using L            = Lay<dim, n>;
     constexpr int nl   = L::nl;
     constexpr int nd   = L::nd;
@@ -582,10 +589,14 @@ namespace dgops
 
     // the cell's own values of the first component are requested first:
     // the x sweep then overlaps the neighbour-index -> trace load chain
-    auto load_own = [&](const T *src, T(&r)[n]) {
+    // (values at the Gauss points of the collocation basis are read along
+    // LAST-direction lines, where the operator starts)
+    constexpr int in_dir = fin == fin_local ? LAST : 0;
+    auto          load_own = [&](const T *src, T(&r)[n]) {
 #pragma unroll
       for (int i = 0; i < n; ++i)
-        r[i] = active ? src[cell * nd + L::template off<0>(p, i)] : (T)0;
+        r[i] =
+          active ? src[cell * nd + L::template off<in_dir>(p, i)] : (T)0;
     };
     // waves that are issuing global loads (the cell's values and traces,
     // the tables) or stores get issue priority, so that requests leave
@@ -595,7 +606,7 @@ namespace dgops
     load_own(io.src, r0);
     // cell vertices: analytic geometry, and JxW of the mass term with
     // tabulated geometry
-    constexpr bool needX = AN || (geo == geo_table && fin == fin_integrate);
+    constexpr bool needX = AN || (geo == geo_table && fin != fin_inverse_mass);
     if constexpr (needX)
       {
         if (active)
@@ -667,6 +678,17 @@ namespace dgops
             r[i] = r0[i];
         else
           load_own(src, r);
+        if constexpr (fin == fin_local)
+          {
+            // already at the Gauss points; the neighbours do not couple
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+              uqs[c][i] = r[i];
+#pragma unroll
+            for (int f = 0; f < 2 * dim; ++f)
+              uplus[c][f] = dplus[c][f] = 0;
+            continue;
+          }
         // neighbours' face nodes for LDS staging (thread (i, j) loads node
         // (t1 = i, t2 = j) of every face, one load per face); stored after
         // the x sweep so that the sweep does not wait for them
@@ -910,7 +932,7 @@ namespace dgops
         for (int i = 0; i < n; ++i)
           {
             cv[LAST][i] = (T)(Bd->w[i] * cl);
-            jxw[i] = fin == fin_integrate ? (T)(Bd->w[i] * wl[LAST] * vol)
+            jxw[i] = fin != fin_inverse_mass ? (T)(Bd->w[i] * wl[LAST] * vol)
                                           : (T)1;
           }
 #pragma unroll
@@ -926,7 +948,7 @@ namespace dgops
         load_tables();
         __builtin_amdgcn_s_setprio(0);
         // JxW of the mass term on this thread's line
-        if (fin == fin_integrate && active && P.has_mass)
+        if (fin != fin_inverse_mass && active && P.has_mass)
           line_jxw<dim, n, LAST, T, L>(X, Bd->xq, Bd->w, p, jxw);
         else
 #pragma unroll
@@ -992,7 +1014,7 @@ namespace dgops
 #pragma unroll
           for (int i = 0; i < n; ++i)
             acc[i] = 0;
-          if (fin == fin_integrate && P.has_mass)
+          if (fin != fin_inverse_mass && P.has_mass)
             {
 #pragma unroll
               for (int b = 0; b < NC; ++b)
@@ -1018,7 +1040,7 @@ namespace dgops
             }
           if constexpr (diff)
             diffusion_cell_term<dim, n, LAST, T>(P, Bd, B, p, uqs[c], acc);
-          face_terms<dim, n, LAST, T, diff>(P, Bd, B, cell, p, active, nb,
+          face_terms<dim, n, LAST, T, diff, fin == fin_local>(P, Bd, B, cell, p, active, nb,
                                             uplus[c], dplus[c], uqs[c], acc,
                                             vf, fan);
           lds_write_line<dim, n, LAST>(u_l, p, uqs[c]);
@@ -1086,7 +1108,7 @@ namespace dgops
               }
             if constexpr (diff)
               diffusion_cell_term<dim, n, 1, T>(P, Bd, B, p, uq, acc);
-            face_terms<dim, n, 1, T, diff>(P, Bd, B, cell, p, active, nb,
+            face_terms<dim, n, 1, T, diff, fin == fin_local>(P, Bd, B, cell, p, active, nb,
                                            uplus[c], dplus[c], uq, acc, vf,
                                            fan);
             lds_write_line<dim, n, 1>(acc_l, p, acc);
@@ -1124,11 +1146,25 @@ namespace dgops
             }
           if constexpr (diff)
             diffusion_cell_term<dim, n, 0, T>(P, Bd, B, p, uq, acc);
-          face_terms<dim, n, 0, T, diff>(P, Bd, B, cell, p, active, nb,
+          face_terms<dim, n, 0, T, diff, fin == fin_local>(P, Bd, B, cell, p, active, nb,
                                          uplus[c], dplus[c], uq, acc, vf,
                                          fan);
         }
         __syncthreads();
+        if constexpr (fin == fin_local)
+          {
+            // the result stays at the Gauss points, stored from x-lines
+            __builtin_amdgcn_s_setprio(3);
+            if (active)
+              {
+                const int64_t base = c * io.comp_stride + cell * nd;
+#pragma unroll
+                for (int i = 0; i < n; ++i)
+                  io.dst[base + L::template off<0>(p, i)] = io.scale * acc[i];
+              }
+            __builtin_amdgcn_s_setprio(0);
+            continue;
+          }
         if constexpr (fin == fin_inverse_mass)
           {
             T ij[n];

@@ -9,6 +9,7 @@
 #include <map>
 
 #include "burgers_kernel.hpp"
+#include "colloc_kernel.hpp"
 #include "fdm_kernel.hpp"
 #include "mass_kernel.hpp"
 #include "op_kernel.hpp"
@@ -165,6 +166,86 @@ namespace dgops
                        dim3((unsigned)blocks), dim3(CPB * nl), 0, stream, P,
                        (const DevBasis<T> *)bt, (const DevBasis<double> *)bd,
                        (T *)vec);
+    return hipGetLastError();
+  }
+
+  // batched cell solver of the IRK preconditioner: nodal -> collocation
+  // values, and the per-cell GMRES update with the transform back
+  using ToCollocLaunch = hipError_t (*)(int64_t n_cells, int64_t block,
+                                        int ns, const void *basis,
+                                        const void *src, void *dst,
+                                        hipStream_t);
+  using CellGmresLaunch = hipError_t (*)(int64_t n_cells, int64_t block,
+                                         const void *basis, const void *r,
+                                         const void *y, const void *z,
+                                         void *dst, hipStream_t);
+
+  inline uint32_t
+  colloc_key(int dim, int n, int fp32)
+  {
+    return (uint32_t)dim | (uint32_t)n << 2 | (uint32_t)fp32 << 6 | 1u << 23;
+  }
+
+  inline uint32_t
+  cell_gmres_key(int dim, int n, int ns, int fp32)
+  {
+    return (uint32_t)dim | (uint32_t)n << 2 | (uint32_t)ns << 6 |
+           (uint32_t)fp32 << 10 | 1u << 24;
+  }
+
+  std::map<uint32_t, ToCollocLaunch>  &colloc_registry();
+  std::map<uint32_t, CellGmresLaunch> &cell_gmres_registry();
+
+  struct CollocRegistrar
+  {
+    CollocRegistrar(uint32_t key, ToCollocLaunch f)
+    {
+      colloc_registry()[key] = f;
+    }
+  };
+
+  struct CellGmresRegistrar
+  {
+    CellGmresRegistrar(uint32_t key, CellGmresLaunch f)
+    {
+      cell_gmres_registry()[key] = f;
+    }
+  };
+
+  template <int dim, int n, typename T>
+  hipError_t
+  launch_to_colloc(int64_t n_cells, int64_t block, int ns, const void *bt,
+                   const void *src, void *dst, hipStream_t stream)
+  {
+    constexpr int CPB = cells_per_block<dim, n>();
+    constexpr int nl  = Lay<dim, n>::nl;
+    static_assert(CPB * nl <= 128, "kernels are bounded to 128 threads");
+    const int64_t blocks = (n_cells + CPB - 1) / CPB;
+    if (blocks <= 0 || ns <= 0)
+      return hipSuccess;
+    hipLaunchKernelGGL((to_collocation_kernel<dim, n, T>),
+                       dim3((unsigned)blocks, (unsigned)ns), dim3(CPB * nl),
+                       0, stream, n_cells, block, (const DevBasis<T> *)bt,
+                       (const T *)src, (T *)dst);
+    return hipGetLastError();
+  }
+
+  template <int dim, int n, int NS, typename T>
+  hipError_t
+  launch_cell_gmres(int64_t n_cells, int64_t block, const void *bt,
+                    const void *r, const void *y, const void *z, void *dst,
+                    hipStream_t stream)
+  {
+    constexpr int CPB = cells_per_block<dim, n>();
+    constexpr int nl  = Lay<dim, n>::nl;
+    static_assert(CPB * nl <= 128, "kernels are bounded to 128 threads");
+    const int64_t blocks = (n_cells + CPB - 1) / CPB;
+    if (blocks <= 0)
+      return hipSuccess;
+    hipLaunchKernelGGL((cell_gmres_update_kernel<dim, n, NS, T>),
+                       dim3((unsigned)blocks), dim3(CPB * nl), 0, stream,
+                       n_cells, block, (const DevBasis<T> *)bt,
+                       (const T *)r, (const T *)y, (const T *)z, (T *)dst);
     return hipGetLastError();
   }
 

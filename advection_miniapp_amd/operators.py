@@ -196,6 +196,21 @@ class AdvectionOperation:
         L.check(self._lib.dg_irk_precondition_block_jacobi(
             self._h, self._v(dst, "dst", s), self._v(src, "src", s)))
 
+    def precondition_cell_solver_irk(self, dst, src):
+        """Block-Jacobi step with the batched cell solver (one GMRES step
+        per cell in the collocation basis, advection_solver_irk_real.cc:
+        2108-2126)."""
+        s = self.n_stages
+        L.check(self._lib.dg_irk_precondition_cell_solver(
+            self._h, self._v(dst, "dst", s), self._v(src, "src", s)))
+
+    def cell_operator_irk(self, dst, src):
+        """The cell-local stage operator at the Gauss points (values at the
+        Gauss points in and out, neighbour traces zero)."""
+        s = self.n_stages
+        L.check(self._lib.dg_irk_cell_operator(
+            self._h, self._v(dst, "dst", s), self._v(src, "src", s)))
+
     def irk_solve(self, stage, rhs, control):
         """(F)GMRES on the stage system from the initial guess in `stage`
         (SolverFGMRES::solve, advection_solver_irk_real.cc:2729-2757);
@@ -229,7 +244,7 @@ class AdvectionOperation:
 
 def solver_control(max_iterations=400, rel_tolerance=1e-8, abs_tolerance=0.0,
                    restart=10, flexible=True, prec_precision=L.DG_FP64,
-                   inner_iterations=0):
+                   inner_iterations=0, cell_solver=False):
     """dg_solver_control with the reference's defaults
     (SolverControl(400, 1e-8 |rhs|), max_basis_size 10, FGMRES)."""
     c = L.DgSolverControl()
@@ -240,6 +255,7 @@ def solver_control(max_iterations=400, rel_tolerance=1e-8, abs_tolerance=0.0,
     c.flexible = 1 if flexible else 0
     c.prec_precision = prec_precision
     c.inner_iterations = inner_iterations
+    c.cell_solver = 1 if cell_solver else 0
     return c
 
 

@@ -404,6 +404,53 @@ def time_config1_run():
                 rel_l2_error=err / nrm)
 
 
+def time_cell_solver(refine=5, reps=20):
+    """The batched cell solver of the block-Jacobi preconditioner
+    (advection_solver_irk_real.cc:2079-2126) on the IRK workload: one
+    application against the FDM alone, and FGMRES(30) solves with each."""
+    import torch
+    from advection_miniapp_amd import AdvectionOperation, make_config
+    from advection_miniapp_amd import _lib as L
+    from advection_miniapp_amd.operators import solver_control
+    op = AdvectionOperation(make_config(**irk_workload(refine=refine)))
+    s = 3
+    u = op.initialize_dof_vector()
+    op.project_initial(u, 0.0)
+    dt = 5.0 / 2 ** refine / 2.0
+    op.set_time(0.0, dt)
+    rhs = op.initialize_dof_vector(blocks=s)
+    op.compute_rhs(rhs, u)
+    out = {}
+    x = op.initialize_dof_vector(blocks=s)
+    for name, f in (("fdm", op.precondition_block_jacobi_irk),
+                    ("cell_solver", op.precondition_cell_solver_irk)):
+        f(x, rhs)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f(x, rhs)
+        torch.cuda.synchronize()
+        out[f"{name}_apply_ms"] = 1e3 * (time.perf_counter() - t0) / reps
+    for name, kw in (("fp64", dict(prec_precision=L.DG_FP64)),
+                     ("fp32_inner4", dict(prec_precision=L.DG_FP32,
+                                          inner_iterations=4))):
+        for cs in (False, True):
+            times = []
+            for rep in range(3):
+                z = op.initialize_dof_vector(blocks=s)
+                ctl = solver_control(max_iterations=400, rel_tolerance=1e-10,
+                                     restart=30, cell_solver=cs, **kw)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                op.irk_solve(z, rhs, ctl)
+                torch.cuda.synchronize()
+                if rep > 0:
+                    times.append(time.perf_counter() - t0)
+            out[f"{name}_{'cell' if cs else 'fdm'}"] = dict(
+                ms=1e3 * min(times), iterations=ctl.iterations)
+    return out
+
+
 def time_extra():
     """Configuration 3 (advection + SIPG diffusion, 32^3 cells, degrees
     3/5/7, FP64 and FP32 operators, FP32 error against FP64) and the
@@ -434,6 +481,7 @@ def time_extra():
     out["config2_run"] = time_config2_run()
     out["inverse_mass"] = time_inverse_mass()
     out["config1_run"] = time_config1_run()
+    out["cell_solver"] = time_cell_solver()
     bk = dict(dim=3, degree=3, n_components=3, operator_kind=2,
               n_coarse=[3, 3, 3], refine=4, lo=[-1, -1, -1], hi=[1, 1, 1],
               solution_kind=6)

@@ -242,11 +242,28 @@ typedef struct dg_solver_control
   int    flexible;        /* 1: FGMRES, 0: right-preconditioned GMRES    */
   int    prec_precision;  /* dg_precision of the FDM preconditioner       */
   int    inner_iterations; /* >0: FP32 inner GMRES as preconditioner      */
+  int    cell_solver;     /* 1: the block-Jacobi step is the batched cell
+                             solver (do_batched_solver, one GMRES step per
+                             cell in the collocation basis) instead of the
+                             FDM alone       advection_solver_irk_real.cc:2108-2126 */
   /* outputs */
   int    iterations;
   double initial_residual;
   double final_residual;
 } dg_solver_control;
+
+/* Block-Jacobi step with the batched cell solver: per cell one
+ * right-preconditioned GMRES step on the cell-local stage operator in the
+ * Gauss-point collocation basis, preconditioned by the FDM of that basis
+ *         precondition_block_jacobi, do_batched_solver = true,
+ *                                  advection_solver_irk_real.cc:2079-2126 */
+int dg_irk_precondition_cell_solver(dg_operator *op, void *dst,
+                                    const void *src);
+/* The cell-local stage operator of that solver: values at the Gauss
+ * points of every cell (lexicographic) in, the weak form tested with the
+ * Lagrange functions of the Gauss points out, neighbour traces zero
+ *              CellwiseOperator::vmult advection_solver_irk_real.cc:545-711 */
+int dg_irk_cell_operator(dg_operator *op, void *dst, const void *src);
 
 /* Solve the IRK stage system (A^-1 (x) M)/dt Z + (I (x) L) Z = rhs with
  * (F)GMRES and the FDM preconditioner; `stage_blocks` holds the initial
