@@ -468,6 +468,7 @@ namespace dgops
     double
     avg_stage_factor(const Operator &op, double t, double dt)
     {
+      require(op.irk.s > 0, "IRK stage table not set up", DG_ERR_STATE);
       double f = 0;
       for (int c = 0; c < op.irk.s; ++c)
         f += velocity_time_factor(op.cfg, t + op.irk.c[c] * dt);

@@ -51,7 +51,7 @@ def irk_workload(refine=5):
                 periodic=[1, 1, 1], lo=[0, 0, 0], hi=[1, 1, 1])
 
 
-def cpu_baseline(threads=None, target_seconds=15.0):
+def cpu_baseline(threads=None, target_seconds=12.0):
     """The oracle's apply (restated reference algorithm, OpenMP over cells)
     on a bounded sample: 16^3 cells of the apply workload."""
     sys.path.insert(0, ROOT)
@@ -68,7 +68,7 @@ def cpu_baseline(threads=None, target_seconds=15.0):
         o.apply(u, 0.0)
         reps += 1
         el = time.perf_counter() - t0
-        if el > target_seconds or reps >= 200:
+        if el > target_seconds or reps >= 2000:
             break
     return dict(value=o.n_dofs * reps / el, unit="DoF/s", cores=cores,
                 kind="port",
