@@ -1,7 +1,8 @@
 /*
  * CPU oracle: test infrastructure only. A plain restatement of the
  * reference's DG algorithms (deal.II FEEvaluation semantics) used as the
- * parity checker of the GPU library and as the CPU baseline of bench.py.
+ * expected result in the parity tests of the GPU library and as the CPU
+ * baseline of bench.py.
  * Nothing in the product links or calls this library.
  */
 #ifndef DG_ORACLE_H
