@@ -50,6 +50,29 @@ namespace dgops
                           int64_t x_stride, void *y, double *result);
   void vec_copy_convert(Operator &op, int64_t n, int ps, const void *src,
                         int pd, void *dst);
+
+  // Arnoldi steps with their scalars in device memory (d_ pointers), for
+  // a Krylov method without host round trips; m <= 16 everywhere.
+  // d_out[j] = <x_j, y>, j < m
+  void vec_multi_dot_dev(Operator &op, int p, int64_t n, int m,
+                         const void *x, int64_t x_stride, const void *y,
+                         double *d_out);
+  // x *= 1 / sqrt(*d_s2)
+  void vec_scale_inv_sqrt(Operator &op, int p, int64_t n, const double *d_s2,
+                          void *x);
+  // w = (w - sum_{k<m} h_k V_k) / sqrt(h_m - sum h_k^2), with d_h[0..m-1]
+  // = <V_k, w> and d_h[m] = <w, w> (one classical Gram-Schmidt sweep)
+  void vec_arnoldi_update(Operator &op, int p, int64_t n, int m,
+                          const double *d_h, const void *V, int64_t v_stride,
+                          void *w);
+  // d_y = argmin |sqrt(*d_beta2) e1 - H y|, column j of the Hessenberg
+  // matrix at d_h + j h_stride as written by vec_multi_dot_dev for
+  // vec_arnoldi_update
+  void vec_arnoldi_lsq(Operator &op, int m, const double *d_h,
+                       int64_t h_stride, const double *d_beta2, double *d_y);
+  // FP64 x = sum_{k<m} d_y[k] Z_k, Z in precision p
+  void vec_combine(Operator &op, int p, int64_t n, int m, const double *d_y,
+                   const void *Z, int64_t z_stride, double *x);
   // one pass over four vectors and a right-hand side: the lower triangle of
   // their Gram matrix (10 values, row by row) and their products with rhs
   // (4 values), as the fused loop of advection_solver_irk_real.cc:2551-2645

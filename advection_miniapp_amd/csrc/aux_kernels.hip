@@ -454,6 +454,132 @@ namespace dgops
         }
     }
 
+    // ---- Arnoldi steps with their scalars in device memory ------------------
+    // (the fixed-length inner GMRES of the mixed-precision solve runs with
+    // no host round trip)
+
+    constexpr int max_arnoldi = 16;
+
+    // x *= 1 / sqrt(*s2) (x = 0 when *s2 = 0: then x was zero already)
+    template <typename T>
+    __global__ void
+    scale_inv_sqrt_kernel(int64_t n, const double *__restrict__ s2, T *x)
+    {
+      const double v = *s2;
+      const T      f = v > 0 ? (T)(1. / sqrt(v)) : (T)0;
+      for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+           i += (int64_t)gridDim.x * blockDim.x)
+        x[i] *= f;
+    }
+
+    // norm of w - sum_k h_k V_k from h[0..m-1] = <V_k, w> and h[m] = <w, w>
+    // (one classical Gram-Schmidt sweep against orthonormal V_k)
+    __device__ __forceinline__ double
+    arnoldi_norm(const double *__restrict__ h, int m)
+    {
+      double hh = 0;
+      for (int k = 0; k < m; ++k)
+        hh = fma(h[k], h[k], hh);
+      const double r2 = h[m] - hh;
+      return r2 > 0 ? sqrt(r2) : 0.;
+    }
+
+    // w = (w - sum_k h_k V_k) / arnoldi_norm(h, M) (unscaled on breakdown)
+    template <typename T, int M>
+    __global__ void
+    arnoldi_update_kernel(int64_t n, const double *__restrict__ h,
+                          const T *__restrict__ V, int64_t vs, T *w)
+    {
+      T c[M];
+#pragma unroll
+      for (int k = 0; k < M; ++k)
+        c[k] = (T)h[k];
+      const double hn = arnoldi_norm(h, M);
+      const T      f  = hn > 0 ? (T)(1. / hn) : (T)1;
+      for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+           i += (int64_t)gridDim.x * blockDim.x)
+        {
+          T s = w[i];
+#pragma unroll
+          for (int k = 0; k < M; ++k)
+            s = fma(-c[k], V[k * vs + i], s);
+          w[i] = s * f;
+        }
+    }
+
+    // min |beta e1 - H y| for the (m+1) x m Hessenberg matrix whose column
+    // j sits at h + j hs as (H_0j .. H_jj, <w_j, w_j>), the subdiagonal
+    // H_j+1,j = arnoldi_norm of that column; Givens QR and back
+    // substitution by one thread
+    __global__ void
+    arnoldi_lsq_kernel(int m, const double *__restrict__ h, int64_t hs,
+                       const double *__restrict__ beta2, double *y)
+    {
+      __shared__ double H[(max_arnoldi + 1) * max_arnoldi];
+      __shared__ double g[max_arnoldi + 1], cs[max_arnoldi],
+        sn[max_arnoldi];
+      if (threadIdx.x != 0)
+        return;
+      for (int j = 0; j < m; ++j)
+        {
+          const double *col = h + j * hs;
+          for (int k = 0; k <= m; ++k)
+            H[k * max_arnoldi + j] =
+              k <= j ? col[k] : k == j + 1 ? arnoldi_norm(col, j + 1) : 0.;
+        }
+      const double b2 = *beta2;
+      g[0]            = b2 > 0 ? sqrt(b2) : 0.;
+      for (int k = 1; k <= m; ++k)
+        g[k] = 0;
+      for (int j = 0; j < m; ++j)
+        {
+          for (int k = 0; k < j; ++k)
+            {
+              const double a = H[k * max_arnoldi + j],
+                           b = H[(k + 1) * max_arnoldi + j];
+              H[k * max_arnoldi + j]       = cs[k] * a + sn[k] * b;
+              H[(k + 1) * max_arnoldi + j] = -sn[k] * a + cs[k] * b;
+            }
+          const double a = H[j * max_arnoldi + j],
+                       b = H[(j + 1) * max_arnoldi + j];
+          const double r = sqrt(a * a + b * b);
+          cs[j]          = r == 0 ? 1. : a / r;
+          sn[j]          = r == 0 ? 0. : b / r;
+          H[j * max_arnoldi + j] = r;
+          g[j + 1]               = -sn[j] * g[j];
+          g[j]                   = cs[j] * g[j];
+        }
+      for (int k = m - 1; k >= 0; --k)
+        {
+          double s = g[k];
+          for (int l = k + 1; l < m; ++l)
+            s -= H[k * max_arnoldi + l] * y[l];
+          const double d = H[k * max_arnoldi + k];
+          y[k]           = d != 0 ? s / d : 0.;
+        }
+    }
+
+    // x = sum_k y_k Z_k, accumulated in FP64, stored as TO
+    template <typename TI, typename TO, int M>
+    __global__ void
+    combine_kernel(int64_t n, const double *__restrict__ y,
+                   const TI *__restrict__ Z, int64_t zs, TO *x)
+    {
+      double c[M];
+#pragma unroll
+      for (int k = 0; k < M; ++k)
+        c[k] = y[k];
+      for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+           i += (int64_t)gridDim.x * blockDim.x)
+        {
+          double s = 0;
+#pragma unroll
+          for (int k = 0; k < M; ++k)
+            s = fma(c[k], (double)Z[k * zs + i], s);
+          x[i] = (TO)s;
+        }
+    }
+
     __global__ void
     gram4_kernel(int64_t n, const double *__restrict__ v0,
                  const double *__restrict__ v1, const double *__restrict__ v2,
@@ -568,6 +694,80 @@ namespace dgops
       hip_check(hipStreamSynchronize(op.stream), "sync");
       for (int j = 0; j < m; ++j)
         result[j] = host[j];
+    }
+
+    // multi_dot_t with the sums left in device memory at d_out[0..m-1]
+    template <typename T>
+    void
+    multi_dot_dev_t(Operator &op, int64_t n, int m, const T *x, int64_t xs,
+                    const T *y, double *d_out)
+    {
+      const unsigned nb = std::min<unsigned>(grid_for(n), 1024);
+      op.d_partials.alloc(sizeof(double) * nb * max_multi);
+      double *part = op.d_partials.as<double>();
+      int     done = 0;
+      while (done < m)
+        {
+          const int chunk = std::min(max_multi, m - done);
+          const T  *xc    = x + done * xs;
+#define DG_MD(M)                                                              \
+  case M:                                                                     \
+    hipLaunchKernelGGL((multi_dot_kernel<T, M>), dim3(nb), dim3(vec_threads), \
+                       0, op.stream, n, xc, xs, y, part);                     \
+    break;
+          switch (chunk)
+            {
+              DG_MD(1) DG_MD(2) DG_MD(3) DG_MD(4) DG_MD(5) DG_MD(6) DG_MD(7)
+              DG_MD(8) DG_MD(9) DG_MD(10) DG_MD(11) DG_MD(12) DG_MD(13)
+              DG_MD(14) DG_MD(15) DG_MD(16)
+            }
+#undef DG_MD
+          hip_check(hipGetLastError(), "multi_dot");
+          hipLaunchKernelGGL(reduce_columns, dim3(1), dim3(1024), 0, op.stream,
+                             part, (int64_t)nb, chunk, d_out + done);
+          hip_check(hipGetLastError(), "reduce");
+          done += chunk;
+        }
+    }
+
+    template <typename T>
+    void
+    arnoldi_update_t(Operator &op, int64_t n, int m, const double *d_h,
+                     const T *V, int64_t vs, T *w)
+    {
+#define DG_AU(M)                                                            \
+  case M:                                                                   \
+    hipLaunchKernelGGL((arnoldi_update_kernel<T, M>), dim3(grid_for(n)),    \
+                       dim3(vec_threads), 0, op.stream, n, d_h, V, vs, w);  \
+    break;
+      switch (m)
+        {
+          DG_AU(1) DG_AU(2) DG_AU(3) DG_AU(4) DG_AU(5) DG_AU(6) DG_AU(7)
+          DG_AU(8) DG_AU(9) DG_AU(10) DG_AU(11) DG_AU(12) DG_AU(13)
+          DG_AU(14) DG_AU(15) DG_AU(16)
+        }
+#undef DG_AU
+      hip_check(hipGetLastError(), "arnoldi update");
+    }
+
+    template <typename TI>
+    void
+    combine_t(Operator &op, int64_t n, int m, const double *d_y, const TI *Z,
+              int64_t zs, double *x)
+    {
+#define DG_CB(M)                                                              \
+  case M:                                                                     \
+    hipLaunchKernelGGL((combine_kernel<TI, double, M>), dim3(grid_for(n)),    \
+                       dim3(vec_threads), 0, op.stream, n, d_y, Z, zs, x);    \
+    break;
+      switch (m)
+        {
+          DG_CB(1) DG_CB(2) DG_CB(3) DG_CB(4) DG_CB(5) DG_CB(6) DG_CB(7)
+          DG_CB(8) DG_CB(9) DG_CB(10) DG_CB(11) DG_CB(12) DG_CB(13)
+          DG_CB(14) DG_CB(15) DG_CB(16)
+        }
+#undef DG_CB
+      hip_check(hipGetLastError(), "combine");
     }
 
     template <typename T>
@@ -1099,6 +1299,80 @@ namespace dgops
     else
       multi_dot_t<double>(op, n, m, (const double *)x, xs, (const double *)y,
                           result);
+  }
+
+  void
+  vec_multi_dot_dev(Operator &op, int p, int64_t n, int m, const void *x,
+                    int64_t xs, const void *y, double *d_out)
+  {
+    if (m <= 0)
+      return;
+    if (n <= 0)
+      throw Error(DG_ERR_ARGUMENT, "multi_dot_dev: empty vectors");
+    if (p == DG_FP32)
+      multi_dot_dev_t<float>(op, n, m, (const float *)x, xs,
+                             (const float *)y, d_out);
+    else
+      multi_dot_dev_t<double>(op, n, m, (const double *)x, xs,
+                              (const double *)y, d_out);
+  }
+
+  void
+  vec_scale_inv_sqrt(Operator &op, int p, int64_t n, const double *d_s2,
+                     void *x)
+  {
+    if (n <= 0)
+      return;
+    if (p == DG_FP32)
+      hipLaunchKernelGGL(scale_inv_sqrt_kernel<float>, dim3(grid_for(n)),
+                         dim3(vec_threads), 0, op.stream, n, d_s2,
+                         (float *)x);
+    else
+      hipLaunchKernelGGL(scale_inv_sqrt_kernel<double>, dim3(grid_for(n)),
+                         dim3(vec_threads), 0, op.stream, n, d_s2,
+                         (double *)x);
+    hip_check(hipGetLastError(), "scale");
+  }
+
+  void
+  vec_arnoldi_update(Operator &op, int p, int64_t n, int m, const double *d_h,
+                     const void *V, int64_t vs, void *w)
+  {
+    if (m < 1 || m > max_arnoldi)
+      throw Error(DG_ERR_ARGUMENT, "arnoldi update: 1..16 basis vectors");
+    if (n <= 0)
+      return;
+    if (p == DG_FP32)
+      arnoldi_update_t<float>(op, n, m, d_h, (const float *)V, vs,
+                              (float *)w);
+    else
+      arnoldi_update_t<double>(op, n, m, d_h, (const double *)V, vs,
+                               (double *)w);
+  }
+
+  void
+  vec_arnoldi_lsq(Operator &op, int m, const double *d_h, int64_t hs,
+                  const double *d_beta2, double *d_y)
+  {
+    if (m < 1 || m > max_arnoldi)
+      throw Error(DG_ERR_ARGUMENT, "arnoldi least squares: 1..16 columns");
+    hipLaunchKernelGGL(arnoldi_lsq_kernel, dim3(1), dim3(64), 0, op.stream, m,
+                       d_h, hs, d_beta2, d_y);
+    hip_check(hipGetLastError(), "arnoldi least squares");
+  }
+
+  void
+  vec_combine(Operator &op, int p, int64_t n, int m, const double *d_y,
+              const void *Z, int64_t zs, double *x)
+  {
+    if (m < 1 || m > max_arnoldi)
+      throw Error(DG_ERR_ARGUMENT, "combine: 1..16 vectors");
+    if (n <= 0)
+      return;
+    if (p == DG_FP32)
+      combine_t<float>(op, n, m, d_y, (const float *)Z, zs, x);
+    else
+      combine_t<double>(op, n, m, d_y, (const double *)Z, zs, x);
   }
 
   void

@@ -499,8 +499,7 @@ namespace dgops
     // 1e-8 reduction); its projections come out of the subtraction pass.
     KrylovResult
     gmres(Operator &op, const LinOp &L, void *x, const void *b, int restart,
-          int max_it, double abs_tol, bool flexible, size_t slot0,
-          bool fixed_iterations = false)
+          int max_it, double abs_tol, bool flexible, size_t slot0)
     {
       const int     m   = std::max(1, restart);
       const int64_t n   = L.n;
@@ -522,7 +521,7 @@ namespace dgops
       vec_multi_dot(op, L.prec, n, 1, vec(V, 0), n, vec(V, 0), &beta2);
       double beta = std::sqrt(std::max(beta2, 0.));
       res.initial = res.final = beta;
-      if (!fixed_iterations && beta <= abs_tol)
+      if (beta <= abs_tol)
         {
           res.converged = true;
           return res;
@@ -599,8 +598,7 @@ namespace dgops
               g[j]     = cs[j] * g[j];
               ++total;
               res.final = std::fabs(g[j + 1]);
-              if ((!fixed_iterations && res.final <= abs_tol) ||
-                  total >= max_it || hn == 0.)
+              if (res.final <= abs_tol || total >= max_it || hn == 0.)
                 {
                   ++j;
                   done = true;
@@ -647,6 +645,49 @@ namespace dgops
         }
       res.iterations = total;
       return res;
+    }
+
+    // The inner solver of the mixed-precision method: m steps of
+    // right-preconditioned GMRES from x0 = 0 in the vector precision of L,
+    // in flexible form (z_j = P v_j kept, x = Z y, so no preconditioner
+    // application after the last step), one classical Gram-Schmidt sweep
+    // per step. Every scalar stays in device memory: the whole solve is a
+    // sequence of launches without a host round trip or a convergence
+    // test. b and x are FP64 vectors, converted on the way in and out.
+    void
+    gmres_inner_device(Operator &op, const LinOp &L, void *x, const void *b,
+                       int m, size_t slot0)
+    {
+      require(m >= 1 && m <= 16, "inner GMRES: 1..16 iterations",
+              DG_ERR_UNSUPPORTED);
+      const int64_t n  = L.n;
+      const size_t  vb = (L.prec == DG_FP32 ? 4 : 8) * n;
+      char   *V = (char *)op.solver_vector(slot0, vb * (m + 1));
+      char   *Z = (char *)op.solver_vector(slot0 + 1, vb * m);
+      double *S = (double *)op.solver_vector(
+        slot0 + 2, sizeof(double) * (m * (m + 2) + 1 + m));
+      const int64_t hs    = m + 2; // column j: H_0j .. H_jj, <w_j, w_j>
+      double       *beta2 = S + m * hs;
+      double       *y     = beta2 + 1;
+      auto vec = [&](char *base, int j) { return (void *)(base + vb * j); };
+
+      vec_copy_convert(op, n, DG_FP64, b, L.prec, vec(V, 0));
+      vec_multi_dot_dev(op, L.prec, n, 1, vec(V, 0), n, vec(V, 0), beta2);
+      vec_scale_inv_sqrt(op, L.prec, n, beta2, vec(V, 0));
+      for (int j = 0; j < m; ++j)
+        {
+          L.P(vec(V, j), vec(Z, j));
+          L.A(vec(Z, j), vec(V, j + 1));
+          vec_multi_dot_dev(op, L.prec, n, j + 2, V, n, vec(V, j + 1),
+                            S + j * hs);
+          // the last Krylov vector is not used again: its norm enters
+          // the least-squares problem only
+          if (j + 1 < m)
+            vec_arnoldi_update(op, L.prec, n, j + 1, S + j * hs, V, n,
+                               vec(V, j + 1));
+        }
+      vec_arnoldi_lsq(op, m, S, hs, beta2, y);
+      vec_combine(op, L.prec, n, m, y, Z, n, (double *)x);
     }
   } // namespace
 } // namespace dgops
@@ -837,12 +878,11 @@ dg_irk_solve(dg_operator *h, void *stage, const void *rhs,
     const int pp = ctl->prec_precision;
     if (ctl->inner_iterations > 0)
       {
-        // inner right-preconditioned GMRES in `pp` with the FDM
+        // inner right-preconditioned GMRES in `pp` with the FDM (or the
+        // batched cell solver), a fixed number of steps on the device
         require(ctl->flexible, "an inner solver needs the flexible method");
         const bool f32 = pp == DG_FP32;
-        char *xin = (char *)op->solver_vector(10, (f32 ? 4 : 8) * N);
-        char *bin = (char *)op->solver_vector(11, (f32 ? 4 : 8) * N);
-        outer.P = [&, f32, xin, bin](const void *src, void *dst) {
+        outer.P = [&, f32](const void *src, void *dst) {
           LinOp in;
           in.prec = f32 ? DG_FP32 : DG_FP64;
           in.n    = N;
@@ -855,11 +895,7 @@ dg_irk_solve(dg_operator *h, void *stage, const void *rhs,
             else
               launch_fdm_variant(*op, s, f32 ? 1 : 0, vs, 1. / dt, x, y);
           };
-          vec_copy_convert(*op, N, DG_FP64, src, in.prec, bin);
-          vec_axpby(*op, in.prec, N, 0., bin, 0., xin);
-          gmres(*op, in, xin, bin, ctl->inner_iterations,
-                ctl->inner_iterations, 0., false, 20, true);
-          vec_copy_convert(*op, N, in.prec, xin, DG_FP64, dst);
+          gmres_inner_device(*op, in, dst, src, ctl->inner_iterations, 20);
         };
       }
     else
