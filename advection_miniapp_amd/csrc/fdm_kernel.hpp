@@ -29,6 +29,10 @@
 
 namespace dgops
 {
+  // step-3 tasks per cell: at most 2 stage blocks (s <= 4) times 4^3
+  // direction blocks (n <= 8)
+  constexpr int max_fdm_tasks = 128;
+
   template <typename T>
   struct FdmDev
   {
@@ -46,6 +50,11 @@ namespace dgops
     int n_stage_blocks;
     int stage_block_col[5];
     int stage_block_size[5];
+    // step-3 tasks of one cell, (stage block, direction blocks) packed as
+    // sb | bd0 << 4 | bd1 << 8 | bd2 << 12, sorted by their number of
+    // paired factors NB: tasks[task_start[NB] .. task_start[NB + 1])
+    int task_start[6];
+    int tasks[max_fdm_tasks];
   };
 
   struct FdmParams
@@ -161,6 +170,62 @@ namespace dgops
       }
   }
 
+  // step 3 for the tasks with NB paired factors of the CPB cells of a
+  // workgroup (task k of cell lc on thread k * CPB + lc, strided by the
+  // workgroup size); s_cell[lc] = (|a_0|, |a_1|, |a_2|, det J^-1)
+  template <int NB, int dim, int n, int NS, int CPB, typename Tc>
+  __device__ __forceinline__ void
+  fdm_pointwise_group(const FdmDev<Tc> &F, Tc *s_data,
+                      const Tc (*s_cell)[4], int64_t cell0, int64_t n_cells,
+                      Tc idt)
+  {
+    constexpr int STR = Lay<dim, n>::stride;
+    const int     t0  = F.task_start[NB];
+    const int     nt  = (F.task_start[NB + 1] - t0) * CPB;
+    for (int t = threadIdx.x; t < nt; t += blockDim.x)
+      {
+        const int k  = t / CPB;
+        const int lc = t - k * CPB;
+        if (cell0 + lc >= n_cells)
+          continue;
+        const int code  = F.tasks[t0 + k];
+        const int sb    = code & 15;
+        const int bd[3] = {(code >> 4) & 15, (code >> 8) & 15,
+                           (code >> 12) & 15};
+        // factor 0: stage, 1..dim: directions
+        const int js       = F.stage_block_col[sb];
+        int       base     = js * STR;
+        Tc        lr       = F.mre[js] * idt;
+        int       delta[4] = {0, 0, 0, 0};
+        Tc        li[4]    = {0, 0, 0, 0};
+        int       nb       = 0;
+        if (F.stage_block_size[sb] == 2)
+          {
+            delta[0] = STR;
+            li[0]    = F.mim[js] * idt;
+            nb       = 1;
+          }
+        int stride = 1;
+#pragma unroll
+        for (int d = 0; d < dim; ++d)
+          {
+            const int c = F.block_col[bd[d]];
+            base += c * stride;
+            lr += s_cell[lc][d] * F.lre[c];
+            if (F.block_size[bd[d]] == 2)
+              {
+                // nb < NB <= 4 here
+                delta[nb] = stride;
+                li[nb]    = s_cell[lc][d] * F.lim[c];
+                ++nb;
+              }
+            stride *= n;
+          }
+        fdm_block<NB, Tc>(s_data + lc * NS * STR, base, delta, lr, li,
+                          s_cell[lc][3]);
+      }
+  }
+
   template <int dim, int n, int NS, typename Tc, typename Tin, typename Tout>
   __global__ void __launch_bounds__(128)
   fdm_kernel(const FdmParams P, const FdmDev<Tc> *__restrict__ Fp,
@@ -185,18 +250,21 @@ namespace dgops
     const int         lcs    = lc < CPB ? lc : 0;
     Tc               *data   = s_data + lcs * NS * STR;
 
-    bool rv[3]   = {false, false, false};
-    Tc   aabs[3] = {0, 0, 0};
-    Tc   invdet  = 1;
+    // |speed| per direction and det J^-1 of the workgroup's cells, read by
+    // any thread in step 3
+    __shared__ Tc s_cell[CPB][4];
+    bool rv[3] = {false, false, false};
     if (active)
       {
         for (int d = 0; d < dim; ++d)
           {
             const double a = P.cell_velocity[cell * dim + d] * P.vel_scale;
-            aabs[d]        = (Tc)fabs(a);
             rv[d]          = a < 0.;
+            if (p == 0)
+              s_cell[lc][d] = (Tc)fabs(a);
           }
-        invdet = (Tc)P.cell_invdet[cell];
+        if (p == 0)
+          s_cell[lc][3] = (Tc)P.cell_invdet[cell];
       }
 
     // 1-2: load (x mirrored), stage transform, V^-1 M^-1 sweeps x, y, (z)
@@ -254,70 +322,24 @@ namespace dgops
       __syncthreads();
     }
 
-    // 3: pointwise inverse on the blocks of this cell
+    // 3: pointwise inverse on the blocks of the workgroup's cells, one
+    // group of block shapes (number of paired factors NB) after the other:
+    // the threads of a wave run one shape at a time
     {
-      const int nbd   = F.n_blocks;
-      const int nspb  = dim == 3 ? nbd * nbd * nbd : nbd * nbd;
-      const int ntask = F.n_stage_blocks * nspb;
-      const Tc  idt   = (Tc)P.inv_dt;
-      if (active)
-        for (int task = p; task < ntask; task += nl)
-          {
-            const int sb = task / nspb;
-            int       rb = task - sb * nspb;
-            int       bd[3];
-            bd[0] = rb % nbd;
-            rb /= nbd;
-            bd[1] = rb % nbd;
-            bd[2] = dim == 3 ? rb / nbd : 0;
-            // factor 0: stage, 1..dim: directions
-            const int js   = F.stage_block_col[sb];
-            int       base = js * STR;
-            Tc        lr   = F.mre[js] * idt;
-            int       delta[4] = {0, 0, 0, 0};
-            Tc        li[4]    = {0, 0, 0, 0};
-            int       nb       = 0;
-            if (F.stage_block_size[sb] == 2)
-              {
-                delta[0] = STR;
-                li[0]    = F.mim[js] * idt;
-                nb       = 1;
-              }
-            int stride = 1;
-#pragma unroll
-            for (int d = 0; d < dim; ++d)
-              {
-                const int c = F.block_col[bd[d]];
-                base += c * stride;
-                lr += aabs[d] * F.lre[c];
-                if (F.block_size[bd[d]] == 2)
-                  {
-                    // nb < 4 here: at most one stage bit and dim bits
-                    delta[nb] = stride;
-                    li[nb]    = aabs[d] * F.lim[c];
-                    ++nb;
-                  }
-                stride *= n;
-              }
-            switch (nb)
-              {
-                case 0:
-                  fdm_block<0, Tc>(data, base, delta, lr, li, invdet);
-                  break;
-                case 1:
-                  fdm_block<1, Tc>(data, base, delta, lr, li, invdet);
-                  break;
-                case 2:
-                  fdm_block<2, Tc>(data, base, delta, lr, li, invdet);
-                  break;
-                case 3:
-                  fdm_block<3, Tc>(data, base, delta, lr, li, invdet);
-                  break;
-                default:
-                  fdm_block<4, Tc>(data, base, delta, lr, li, invdet);
-                  break;
-              }
-          }
+      const Tc      idt   = (Tc)P.inv_dt;
+      const int64_t cell0 = cell - lc;
+      fdm_pointwise_group<0, dim, n, NS, CPB>(F, s_data, s_cell, cell0,
+                                              P.n_cells, idt);
+      fdm_pointwise_group<1, dim, n, NS, CPB>(F, s_data, s_cell, cell0,
+                                              P.n_cells, idt);
+      fdm_pointwise_group<2, dim, n, NS, CPB>(F, s_data, s_cell, cell0,
+                                              P.n_cells, idt);
+      if constexpr (dim == 3 || NS > 1)
+        fdm_pointwise_group<3, dim, n, NS, CPB>(F, s_data, s_cell, cell0,
+                                                P.n_cells, idt);
+      if constexpr (dim == 3 && NS > 1)
+        fdm_pointwise_group<4, dim, n, NS, CPB>(F, s_data, s_cell, cell0,
+                                                P.n_cells, idt);
       __syncthreads();
     }
 

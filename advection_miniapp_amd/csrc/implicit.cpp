@@ -194,6 +194,39 @@ namespace dgops
       F.n_stage_blocks = nb;
     }
 
+    // the step-3 tasks of one cell sorted by their number of paired factors
+    // (see FdmDev::tasks)
+    template <typename T>
+    void
+    fill_tasks(FdmDev<T> &F, int dim)
+    {
+      std::vector<int> by_nb[5];
+      const int        nbd  = F.n_blocks;
+      const int        nspb = dim == 3 ? nbd * nbd * nbd : nbd * nbd;
+      for (int sb = 0; sb < F.n_stage_blocks; ++sb)
+        for (int rb = 0; rb < nspb; ++rb)
+          {
+            const int bd[3] = {rb % nbd, (rb / nbd) % nbd,
+                               dim == 3 ? rb / (nbd * nbd) : 0};
+            int       nb    = F.stage_block_size[sb] == 2 ? 1 : 0;
+            for (int d = 0; d < dim; ++d)
+              nb += F.block_size[bd[d]] == 2 ? 1 : 0;
+            by_nb[nb].push_back(sb | bd[0] << 4 | bd[1] << 8 | bd[2] << 12);
+          }
+      int k = 0;
+      for (int nb = 0; nb <= 4; ++nb)
+        {
+          F.task_start[nb] = k;
+          for (const int code : by_nb[nb])
+            {
+              require(k < max_fdm_tasks, "FDM task table overflow",
+                      DG_ERR_UNSUPPORTED);
+              F.tasks[k++] = code;
+            }
+        }
+      F.task_start[5] = k;
+    }
+
     // 1D upwind-DG matrices of the + velocity sign and their real-pair
     // eigendecomposition (advection_solver_irk_real.cc:1490-1541); the
     // - sign is its mirror image (see fdm_kernel.hpp), cell data
@@ -274,6 +307,11 @@ namespace dgops
           hd[2 * colloc + 1] = fdS;
           hf[2 * colloc]     = ff1;
           hf[2 * colloc + 1] = ffS;
+        }
+      for (int i = 0; i < 4; ++i)
+        {
+          fill_tasks(hd[i], op.cfg.dim);
+          fill_tasks(hf[i], op.cfg.dim);
         }
       op.d_fdm_d.alloc(sizeof(hd));
       op.d_fdm_f.alloc(sizeof(hf));
