@@ -33,6 +33,17 @@ namespace dgops
   // direction blocks (n <= 8)
   constexpr int max_fdm_tasks = 128;
 
+  // one step-3 task: a block of real-pair-coupled points of one cell
+  template <typename T>
+  struct FdmTask
+  {
+    // bits 0-1: stage block column, bit 2: stage pair; direction d: bits
+    // 3+4d .. 5+4d block column, bit 6+4d pair
+    int code;
+    T   re[4]; // eigenvalue real parts: stage (times 1/dt), directions
+    T   im[4]; // (times |a_d|); imaginary parts, 0 for real blocks
+  };
+
   template <typename T>
   struct FdmDev
   {
@@ -50,11 +61,10 @@ namespace dgops
     int n_stage_blocks;
     int stage_block_col[5];
     int stage_block_size[5];
-    // step-3 tasks of one cell, (stage block, direction blocks) packed as
-    // sb | bd0 << 4 | bd1 << 8 | bd2 << 12, sorted by their number of
-    // paired factors NB: tasks[task_start[NB] .. task_start[NB + 1])
-    int task_start[6];
-    int tasks[max_fdm_tasks];
+    // step-3 tasks of one cell sorted by their number of paired factors
+    // NB: tasks[task_start[NB] .. task_start[NB + 1])
+    int        task_start[6];
+    FdmTask<T> tasks[max_fdm_tasks];
   };
 
   struct FdmParams
@@ -131,18 +141,31 @@ namespace dgops
             re[e1]       = (T)0.5 * (r0 - i1);
             im[e1]       = (T)0.5 * (i0 + r1);
           }
+    // times invdet / (lr + i lim_e), lim_e = sum_b +-li[b]: the complement
+    // point (every pair bit flipped) has -lim_e and shares the denominator,
+    // so one division serves both
+    constexpr int NH = NB == 0 ? 1 : NE / 2;
 #pragma unroll
-    for (int e = 0; e < NE; ++e)
+    for (int e = 0; e < NH; ++e)
       {
         T lim = 0;
 #pragma unroll
         for (int b = 0; b < NB; ++b)
           lim += ((e >> b) & 1) ? -li[b] : li[b];
-        const T den = lr * lr + lim * lim;
-        const T sr = invdet * lr / den, si = -invdet * lim / den;
-        const T r = re[e], i = im[e];
-        re[e]     = r * sr - i * si;
-        im[e]     = r * si + i * sr;
+        const T q  = invdet / (lr * lr + lim * lim);
+        const T sr = lr * q, si = -lim * q;
+        {
+          const T r = re[e], i = im[e];
+          re[e]     = r * sr - i * si;
+          im[e]     = r * si + i * sr;
+        }
+        if constexpr (NB > 0)
+          {
+            const int ec = e ^ (NE - 1);
+            const T   r  = re[ec], i = im[ec];
+            re[ec]       = r * sr + i * si;
+            im[ec]       = i * sr - r * si;
+          }
       }
     // U on every pair bit: (y+, y-) -> (y+ + y-, i (y+ - y-))
 #pragma unroll
@@ -188,35 +211,34 @@ namespace dgops
         const int lc = t - k * CPB;
         if (cell0 + lc >= n_cells)
           continue;
-        const int code  = F.tasks[t0 + k];
-        const int sb    = code & 15;
-        const int bd[3] = {(code >> 4) & 15, (code >> 8) & 15,
-                           (code >> 12) & 15};
+        // one level of independent loads: the task record
+        const FdmTask<Tc> &tk   = F.tasks[t0 + k];
+        const int          code = tk.code;
+        const Tc m[4] = {idt, s_cell[lc][0], s_cell[lc][1], s_cell[lc][2]};
         // factor 0: stage, 1..dim: directions
-        const int js       = F.stage_block_col[sb];
-        int       base     = js * STR;
-        Tc        lr       = F.mre[js] * idt;
-        int       delta[4] = {0, 0, 0, 0};
-        Tc        li[4]    = {0, 0, 0, 0};
-        int       nb       = 0;
-        if (F.stage_block_size[sb] == 2)
+        int base     = (code & 3) * STR;
+        Tc  lr       = tk.re[0] * m[0];
+        int delta[4] = {0, 0, 0, 0};
+        Tc  li[4]    = {0, 0, 0, 0};
+        int nb       = 0;
+        if (code & 4)
           {
             delta[0] = STR;
-            li[0]    = F.mim[js] * idt;
+            li[0]    = tk.im[0] * m[0];
             nb       = 1;
           }
         int stride = 1;
 #pragma unroll
         for (int d = 0; d < dim; ++d)
           {
-            const int c = F.block_col[bd[d]];
-            base += c * stride;
-            lr += s_cell[lc][d] * F.lre[c];
-            if (F.block_size[bd[d]] == 2)
+            const int f = code >> (3 + 4 * d);
+            base += (f & 7) * stride;
+            lr += tk.re[1 + d] * m[1 + d];
+            if (f & 8)
               {
                 // nb < NB <= 4 here
                 delta[nb] = stride;
-                li[nb]    = s_cell[lc][d] * F.lim[c];
+                li[nb]    = tk.im[1 + d] * m[1 + d];
                 ++nb;
               }
             stride *= n;

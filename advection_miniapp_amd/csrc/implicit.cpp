@@ -194,34 +194,50 @@ namespace dgops
       F.n_stage_blocks = nb;
     }
 
-    // the step-3 tasks of one cell sorted by their number of paired factors
-    // (see FdmDev::tasks)
+    // the step-3 task records of one cell, sorted by their number of paired
+    // factors (see FdmDev::tasks)
     template <typename T>
     void
     fill_tasks(FdmDev<T> &F, int dim)
     {
-      std::vector<int> by_nb[5];
-      const int        nbd  = F.n_blocks;
-      const int        nspb = dim == 3 ? nbd * nbd * nbd : nbd * nbd;
+      std::vector<FdmTask<T>> by_nb[5];
+      const int               nbd  = F.n_blocks;
+      const int               nspb = dim == 3 ? nbd * nbd * nbd : nbd * nbd;
       for (int sb = 0; sb < F.n_stage_blocks; ++sb)
         for (int rb = 0; rb < nspb; ++rb)
           {
-            const int bd[3] = {rb % nbd, (rb / nbd) % nbd,
-                               dim == 3 ? rb / (nbd * nbd) : 0};
-            int       nb    = F.stage_block_size[sb] == 2 ? 1 : 0;
+            const int  bd[3] = {rb % nbd, (rb / nbd) % nbd,
+                                dim == 3 ? rb / (nbd * nbd) : 0};
+            FdmTask<T> tk    = {};
+            const int  js    = F.stage_block_col[sb];
+            const bool sp    = F.stage_block_size[sb] == 2;
+            require(js < 4 && F.n_blocks <= max_n, "FDM task encoding",
+                    DG_ERR_UNSUPPORTED);
+            tk.code  = js | (sp ? 4 : 0);
+            tk.re[0] = F.mre[js];
+            tk.im[0] = sp ? F.mim[js] : (T)0;
+            int nb   = sp ? 1 : 0;
             for (int d = 0; d < dim; ++d)
-              nb += F.block_size[bd[d]] == 2 ? 1 : 0;
-            by_nb[nb].push_back(sb | bd[0] << 4 | bd[1] << 8 | bd[2] << 12);
+              {
+                const int  c  = F.block_col[bd[d]];
+                const bool pr = F.block_size[bd[d]] == 2;
+                require(c < 8, "FDM task encoding", DG_ERR_UNSUPPORTED);
+                tk.code |= (c | (pr ? 8 : 0)) << (3 + 4 * d);
+                tk.re[1 + d] = F.lre[c];
+                tk.im[1 + d] = pr ? F.lim[c] : (T)0;
+                nb += pr ? 1 : 0;
+              }
+            by_nb[nb].push_back(tk);
           }
       int k = 0;
       for (int nb = 0; nb <= 4; ++nb)
         {
           F.task_start[nb] = k;
-          for (const int code : by_nb[nb])
+          for (const auto &tk : by_nb[nb])
             {
               require(k < max_fdm_tasks, "FDM task table overflow",
                       DG_ERR_UNSUPPORTED);
-              F.tasks[k++] = code;
+              F.tasks[k++] = tk;
             }
         }
       F.task_start[5] = k;
