@@ -241,7 +241,8 @@ typedef struct dg_solver_control
   int    restart;         /* max_basis_size                              */
   int    flexible;        /* 1: FGMRES, 0: right-preconditioned GMRES    */
   int    prec_precision;  /* dg_precision of the FDM preconditioner       */
-  int    inner_iterations; /* >0: FP32 inner GMRES as preconditioner      */
+  int    inner_iterations; /* 1..16: inner GMRES of that many steps as
+                             preconditioner, in prec_precision; 0: none */
   int    cell_solver;     /* 1: the block-Jacobi step is the batched cell
                              solver (do_batched_solver, one GMRES step per
                              cell in the collocation basis) instead of the
