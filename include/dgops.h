@@ -117,7 +117,8 @@ typedef struct dg_config
   /* analytic solution: initial data and Dirichlet boundary values        */
   int      solution_kind;
   double   solution_param[8];
-  /* implicit Runge-Kutta stages for the IRK entry points (1..5)          */
+  /* Radau IIA stages of the IRK entry points (1..4; dg_irk_coefficients
+     alone also tabulates 5)                                               */
   int      irk_stages;
 } dg_config;
 
