@@ -918,6 +918,8 @@ dg_irk_solve(dg_operator *h, void *stage, const void *rhs,
             DG_ERR_STATE);
     require(op->time_step != 0., "irk solve: set_time with dt != 0",
             DG_ERR_STATE);
+    require(ctl->inner_iterations >= 0 && ctl->inner_iterations <= 16,
+            "inner GMRES: 0..16 iterations", DG_ERR_UNSUPPORTED);
     setup_irk(*op);
     const int     s  = op->irk.s;
     const int64_t N  = op->n_dofs * s;
