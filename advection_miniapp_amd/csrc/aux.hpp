@@ -41,6 +41,10 @@ namespace dgops
   void vec_multi_axpy(Operator &op, int p, int64_t n, int m,
                       const double *alpha, const void *x, int64_t x_stride,
                       void *y);
+  // y = (y + sum_j alpha_j x_j) * scale
+  void vec_multi_axpy_scaled(Operator &op, int p, int64_t n, int m,
+                             const double *alpha, const void *x,
+                             int64_t x_stride, void *y, double scale);
   void vec_multi_dot(Operator &op, int p, int64_t n, int m, const void *x,
                      int64_t x_stride, const void *y, double *result);
   // y += sum_j alpha_j x_j, then result[j] = <x_j, y> (j < m) and

@@ -342,10 +342,11 @@ namespace dgops
         y[i] = (b == (T)0) ? a * x[i] : fma(a, x[i], b * y[i]);
     }
 
+    // y = (y + sum_j c_j x_j) * scale
     template <typename T, int M>
     __global__ void
     multi_axpy_kernel(int64_t n, const Coeffs c, const T *__restrict__ x,
-                      int64_t xs, T *y)
+                      int64_t xs, T *y, T scale)
     {
       for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
            i += (int64_t)gridDim.x * blockDim.x)
@@ -354,7 +355,26 @@ namespace dgops
 #pragma unroll
           for (int j = 0; j < M; ++j)
             s = fma((T)c.v[j], x[j * xs + i], s);
-          y[i] = s;
+          y[i] = s * scale;
+        }
+    }
+
+    // column sums of partials[count][k] for count <= 1024: one wavefront
+    // per column, no barrier (the block partials of the vector kernels)
+    __global__ void
+    reduce_columns_small(const double *partials, int count, int k,
+                         double *out)
+    {
+      const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+      for (int c = wave; c < k; c += blockDim.x / 64)
+        {
+          double s = 0;
+          for (int i = lane; i < count; i += 64)
+            s += partials[i * k + c];
+          for (int off = 32; off > 0; off >>= 1)
+            s += __shfl_down(s, off, 64);
+          if (lane == 0)
+            out[c] = s;
         }
     }
 
@@ -681,8 +701,8 @@ namespace dgops
             }
 #undef DG_MD
           hip_check(hipGetLastError(), "multi_dot");
-          hipLaunchKernelGGL(reduce_columns, dim3(1), dim3(1024), 0, op.stream,
-                             part, (int64_t)nb, chunk,
+          hipLaunchKernelGGL(reduce_columns_small, dim3(1), dim3(1024), 0,
+                             op.stream, part, (int)nb, chunk,
                              op.d_result.as<double>() + done);
           hip_check(hipGetLastError(), "reduce");
           done += chunk;
@@ -723,8 +743,8 @@ namespace dgops
             }
 #undef DG_MD
           hip_check(hipGetLastError(), "multi_dot");
-          hipLaunchKernelGGL(reduce_columns, dim3(1), dim3(1024), 0, op.stream,
-                             part, (int64_t)nb, chunk, d_out + done);
+          hipLaunchKernelGGL(reduce_columns_small, dim3(1), dim3(1024), 0,
+                             op.stream, part, (int)nb, chunk, d_out + done);
           hip_check(hipGetLastError(), "reduce");
           done += chunk;
         }
@@ -799,8 +819,9 @@ namespace dgops
         }
 #undef DG_MAD
       hip_check(hipGetLastError(), "multi_axpy_dot");
-      hipLaunchKernelGGL(reduce_columns, dim3(1), dim3(1024), 0, op.stream,
-                         part, (int64_t)nb, m + 1, op.d_result.as<double>());
+      hipLaunchKernelGGL(reduce_columns_small, dim3(1), dim3(1024), 0,
+                         op.stream, part, (int)nb, m + 1,
+                         op.d_result.as<double>());
       hip_check(hipGetLastError(), "reduce");
       std::vector<double> host(m + 1);
       hip_check(hipMemcpyAsync(host.data(), op.d_result.ptr,
@@ -812,10 +833,12 @@ namespace dgops
         result[j] = host[j];
     }
 
+    // y = (y + sum_j alpha_j x_j) * scale, in chunks of max_multi vectors
+    // (the scale goes with the last chunk)
     template <typename T>
     void
     multi_axpy_t(Operator &op, int64_t n, int m, const double *alpha,
-                 const T *x, int64_t xs, T *y)
+                 const T *x, int64_t xs, T *y, double scale = 1.)
     {
       int done = 0;
       while (done < m)
@@ -825,10 +848,11 @@ namespace dgops
           for (int j = 0; j < chunk; ++j)
             c.v[j] = alpha[done + j];
           const T *xc = x + done * xs;
+          const T  sc = done + chunk == m ? (T)scale : (T)1;
 #define DG_MA(M)                                                             \
   case M:                                                                    \
     hipLaunchKernelGGL((multi_axpy_kernel<T, M>), dim3(grid_for(n)),         \
-                       dim3(vec_threads), 0, op.stream, n, c, xc, xs, y);    \
+                       dim3(vec_threads), 0, op.stream, n, c, xc, xs, y, sc);\
     break;
           switch (chunk)
             {
@@ -1250,6 +1274,21 @@ namespace dgops
     else
       multi_axpy_t<double>(op, n, m, alpha, (const double *)x, xs,
                            (double *)y);
+  }
+
+  void
+  vec_multi_axpy_scaled(Operator &op, int p, int64_t n, int m,
+                        const double *alpha, const void *x, int64_t xs,
+                        void *y, double scale)
+  {
+    if (n <= 0 || m <= 0)
+      return;
+    if (p == DG_FP32)
+      multi_axpy_t<float>(op, n, m, alpha, (const float *)x, xs, (float *)y,
+                          scale);
+    else
+      multi_axpy_t<double>(op, n, m, alpha, (const double *)x, xs,
+                           (double *)y, scale);
   }
 
   void

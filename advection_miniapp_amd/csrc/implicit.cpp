@@ -611,9 +611,7 @@ namespace dgops
               for (int k = 0; k <= j; ++k)
                 neg[k] = -h[k];
               double hn2 = ww - hh;
-              if (hn2 >= 0.5 * ww)
-                vec_multi_axpy(op, L.prec, n, j + 1, neg.data(), V, n, w);
-              else
+              if (hn2 < 0.5 * ww)
                 {
                   // w -= V h with h2 = V^T w, |w|^2 of the result
                   vec_multi_axpy_dot(op, L.prec, n, j + 1, neg.data(), V, n,
@@ -623,18 +621,19 @@ namespace dgops
                       neg[k] = -h2[k];
                       h[k] += h2[k];
                     }
-                  vec_multi_axpy(op, L.prec, n, j + 1, neg.data(), V, n, w);
                   double s2 = 0;
                   for (int k = 0; k <= j; ++k)
                     s2 += h2[k] * h2[k];
                   hn2 = h2[j + 1] - s2;
                 }
+              // the last subtraction and the normalization in one pass:
+              // |w - V c|^2 = |w|^2 - |c|^2 is known before it
               const double hn = std::sqrt(std::max(hn2, 0.));
+              vec_multi_axpy_scaled(op, L.prec, n, j + 1, neg.data(), V, n, w,
+                                    hn > 0 ? 1. / hn : 1.);
               for (int k = 0; k <= j; ++k)
                 H[k * m + j] = h[k];
               H[(j + 1) * m + j] = hn;
-              if (hn > 0)
-                vec_axpby(op, L.prec, n, 1. / hn, w, 0., w);
               // Givens rotations
               for (int k = 0; k < j; ++k)
                 {
