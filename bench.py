@@ -453,6 +453,61 @@ def time_cell_solver(refine=5, reps=20):
     return out
 
 
+def time_implicit_parts(steps=20):
+    """The single-stage implicit operator (M/dt + L) on the apply workload
+    (advection_solver_implicit_real.cc:876-1058), and Radau IIA(3) time
+    steps with the initial-guess projection on the IRK workload
+    (advection_solver_irk_real.cc:2484-2779): per-step time and outer
+    iterations once the projection has four previous stage solutions."""
+    import torch
+    from advection_miniapp_amd import AdvectionOperation, make_config
+    from advection_miniapp_amd import _lib as L
+    from advection_miniapp_amd.operators import (ImplicitRungeKuttaIntegrator,
+                                                 solver_control)
+    out = {}
+    op = AdvectionOperation(make_config(**dict(apply_workload(),
+                                               factor_skew=0.5)))
+    src = op.initialize_dof_vector()
+    op.project_initial(src, 0.0)
+    dst = op.initialize_dof_vector()
+    op.set_time(0.0, 1e-3)
+    for _ in range(3):
+        op.vmult(dst, src)
+    start, stop = _events()
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(steps):
+        op.vmult(dst, src)
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / steps
+    out["implicit_vmult"] = dict(n_dofs=op.n_dofs, ms=ms,
+                                 dof_per_s=op.n_dofs / (ms * 1e-3))
+    del op, src, dst
+    torch.cuda.empty_cache()
+    refine = 5
+    op = AdvectionOperation(make_config(**irk_workload(refine)))
+    u = op.initialize_dof_vector()
+    op.project_initial(u, 0.0)
+    integ = ImplicitRungeKuttaIntegrator(op)
+    dt = 5.0 / 2 ** refine / 2.0
+    rows = []
+    for step in range(8):
+        ctl = solver_control(max_iterations=400, rel_tolerance=1e-10,
+                             restart=30, prec_precision=L.DG_FP32,
+                             inner_iterations=8)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        integ.perform_time_step(step * dt, dt, u, ctl)
+        torch.cuda.synchronize()
+        rows.append((1e3 * (time.perf_counter() - t0), ctl.iterations))
+    out["irk_steps"] = dict(
+        workload="Radau IIA(3) steps, 32^3 cells, degree 4, Courant 5, "
+                 "FP32 inner GMRES(8), 1e-10",
+        ms_and_iterations_per_step=rows)
+    return out
+
+
 def time_extra():
     """Configuration 3 (advection + SIPG diffusion, 32^3 cells, degrees
     3/5/7, FP64 and FP32 operators, FP32 error against FP64) and the
@@ -484,6 +539,7 @@ def time_extra():
     out["inverse_mass"] = time_inverse_mass()
     out["config1_run"] = time_config1_run()
     out["cell_solver"] = time_cell_solver()
+    out["implicit"] = time_implicit_parts()
     bk = dict(dim=3, degree=3, n_components=3, operator_kind=2,
               n_coarse=[3, 3, 3], refine=4, lo=[-1, -1, -1], hi=[1, 1, 1],
               solution_kind=6)
