@@ -8,9 +8,7 @@ with alpha minimizing |b - alpha A x_m|, and alpha = 1 because x_m is
 optimal along its own direction. So one outer step around an inner
 GMRES(m) equals m steps of the outer FGMRES with the same preconditioner
 (advection_solver_irk_real.cc:2725-2757 for the solver setup)."""
-import numpy as np
 import pytest
-import torch
 
 from advection_miniapp_amd import AdvectionOperation
 from advection_miniapp_amd import _lib as L
