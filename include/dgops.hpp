@@ -229,14 +229,23 @@ namespace dgops
       check(dg_apply_inverse_mass_matrix(op_, device_data(dst)));
     }
 
+    // the reference's do_batched_solver switch
+    // (advection_solver_irk_real.cc:124, :2108-2126): the block-Jacobi step
+    // on stage blocks runs one GMRES step per cell in the collocation basis
+    // instead of the FDM alone
+    void set_batched_solver(const bool on) { batched_ = on; }
+
     template <typename V>
     void
     precondition_block_jacobi(V &dst, const V &src) const
     {
       if (dst.size() == info_.vector_size * (std::size_t)stages_ &&
           stages_ > 1)
-        check(dg_irk_precondition_block_jacobi(op_, device_data(dst),
-                                               device_data(src)));
+        check(batched_ ? dg_irk_precondition_cell_solver(op_, device_data(dst),
+                                                         device_data(src)) :
+                         dg_irk_precondition_block_jacobi(op_,
+                                                          device_data(dst),
+                                                          device_data(src)));
       else
         check(dg_precondition_block_jacobi(op_, device_data(dst),
                                            device_data(src)));
@@ -261,7 +270,8 @@ namespace dgops
   private:
     dg_operator *op_ = nullptr;
     dg_info      info_{};
-    int          stages_ = 1;
+    int          stages_  = 1;
+    bool         batched_ = false;
   };
 
   // BlockJacobi wrapper of the reference (advection_solver_implicit_real.cc

@@ -1,8 +1,10 @@
-"""The C++ facade (include/dgops.hpp) as a drop-in: a caller-side templated
-LSRK integrator written against the reference's perform_stage concept
-(tests/cpp/facade_lsrk.cpp, built by __graft_entry__.build()) runs three
-LSRK(5,4) steps on the GPU; the result matches the oracle's
-LowStorageRungeKuttaIntegrator to 1e-12."""
+"""The C++ facade (include/dgops.hpp) as a drop-in, through C++ drivers
+built by __graft_entry__.build(): a caller-side templated LSRK integrator
+written against the reference's perform_stage concept
+(tests/cpp/facade_lsrk.cpp) runs three LSRK(5,4) steps on the GPU and
+matches the oracle's LowStorageRungeKuttaIntegrator to 1e-12; the IRK
+members and the BlockJacobi preconditioner (tests/cpp/facade_irk.cpp)
+match the oracle and the library."""
 import os
 import subprocess
 
@@ -35,3 +37,51 @@ def test_facade_lsrk_driver_matches_oracle(tmp_path):
         o.lsrk_time_step(1, s * dt, dt, u, ri, ki)
     assert u_gpu.size == u.size
     assert rel_diff(u_gpu, u) < 1e-12
+
+
+IRK_DRIVER = os.path.join(HERE, "cpp", "facade_irk")
+
+
+def test_facade_irk_driver(tmp_path):
+    """The IRK side of the facade (tests/cpp/facade_irk.cpp): stage-block
+    vmult, compute_rhs and update_solution against the oracle at 1e-12;
+    BlockJacobi::vmult with and without the do_batched_solver switch equal
+    to the FDM block-Jacobi step and the batched cell solver of the
+    library (both checked against dense and oracle references in
+    test_gpu_implicit.py and test_gpu_cell_solver.py)."""
+    import torch
+    from advection_miniapp_amd import AdvectionOperation
+    assert os.path.exists(IRK_DRIVER), \
+        "tests/cpp/facade_irk is missing: run __graft_entry__.build()"
+    out = tmp_path / "irk.bin"
+    r = subprocess.run([IRK_DRIVER, str(out)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    c = Cfg(dim=3, degree=3, refine=2, perturbation=0.1, velocity_kind=1,
+            factor_skew=0.5, irk_stages=3, solution_kind=3,
+            solution_param=[0.5, 0.5, 0.5, 0.15])
+    o = c.oracle_handle()
+    t, dt, s = 0.1, 0.05, 3
+    u = o.project(0.0)
+    n = u.size
+    got = np.fromfile(out, dtype=np.float64)
+    assert got.size == 4 * s * n + n
+    vm, fdm, cell, rhs, upd = np.split(got, [s * n, 2 * s * n, 3 * s * n,
+                                             4 * s * n])
+    z = o.irk_compute_rhs(u, t, dt)
+    assert rel_diff(rhs, z) < 1e-12
+    assert rel_diff(vm, o.irk_vmult(z, t, dt)) < 1e-12
+    u2 = u.copy()
+    o.irk_update_solution(u2, z, dt)
+    assert rel_diff(upd, u2) < 1e-12
+    op = AdvectionOperation(c.dg())
+    op.set_time(t, dt)
+    zd = torch.from_numpy(rhs).cuda()
+    ref = op.initialize_dof_vector(blocks=s)
+    op.precondition_block_jacobi_irk(ref, zd)
+    assert rel_diff(fdm, ref.cpu().numpy()) < 1e-14
+    op.precondition_cell_solver_irk(ref, zd)
+    assert rel_diff(cell, ref.cpu().numpy()) < 1e-14
+    # the switch changes the step: the cell solver is not the FDM alone on
+    # a perturbed mesh
+    assert rel_diff(cell, fdm) > 1e-6
