@@ -902,15 +902,14 @@ namespace dgops
         Lay<dim, n>::template idx<dim - 1>(p, 0, q);
     }
 
-    // one block per cell: w adj(J) a at the Gauss points, det J at the
-    // points {0, 1/2, 1}^dim, and w (adj(J) a)_d on the lower face of every
+    // one block per cell: w adj(J) a and 1/JxW at the Gauss points
+    // (lexicographic), and w (adj(J) a)_d on the lower face of every
     // direction, indexed by the line p of direction d that ends on it
     template <int dim, int n>
     __global__ void
     geometry_table_kernel(const OpParams P,
                           const DevBasis<double> *__restrict__ B,
-                          double *tab_cell, double *cell_det,
-                          double *tab_face)
+                          double *tab_cell, double *tab_face)
     {
       constexpr int NV = 1 << dim;
       constexpr int nl = Lay<dim, n>::nl, nd = Lay<dim, n>::nd;
@@ -919,15 +918,15 @@ namespace dgops
       for (int e = threadIdx.x; e < NV * dim; e += blockDim.x)
         X[e] = P.cell_vertices[cell * NV * dim + e];
       __syncthreads();
-      double *tc = tab_cell + cell * dim * nd;
+      double *tc = tab_cell + cell * (dim + 1) * nd;
       for (int pt = threadIdx.x; pt < nd; pt += blockDim.x)
         {
           const int q[3] = {pt % n, (pt / n) % n,
                             dim == 3 ? pt / (n * n) : 0};
           double    cv[3], jxw;
           cell_point_data<dim>(P, X, B->xq, B->w, q, cv, jxw);
-          // row d in the line order of direction d: point i of line p at
-          // i * nl + p
+          // row d in the line order of direction d (row dim, read along x,
+          // in that of x): point i of line p at i * nl + p
           for (int d = 0; d < dim; ++d)
             {
               const int i = q[d];
@@ -936,17 +935,7 @@ namespace dgops
                                    : q[0] + n * q[1];
               tc[d * nd + i * nl + p] = cv[d];
             }
-        }
-      // det J at the 3^dim points {0, 1/2, 1}^dim (x fastest), which fix
-      // it on a multilinear cell (degree 2 per direction)
-      constexpr int n3 = dim == 3 ? 27 : 9;
-      for (int e = threadIdx.x; e < n3; e += blockDim.x)
-        {
-          double xi[3] = {0.5 * (e % 3), 0.5 * ((e / 3) % 3),
-                          dim == 3 ? 0.5 * (e / 9) : 0.};
-          double x[3], J[3][3], adj[3][3];
-          multilinear_map<dim>(X, xi, x, J);
-          cell_det[cell * n3 + e] = adjugate<dim>(J, adj);
+          tc[dim * nd + q[0] * nl + q[1] + n * q[2]] = 1. / jxw;
         }
       for (int t = threadIdx.x; t < dim * nl; t += blockDim.x)
         {
@@ -1012,7 +1001,6 @@ namespace dgops
         hipLaunchKernelGGL((geometry_table_kernel<dim, n>),
                            dim3(op.mesh.n_cells), dim3(aux_threads), 0,
                            op.stream, P, B, op.d_tab_cell[0].as<double>(),
-                           op.d_cell_det[0].as<double>(),
                            op.d_tab_face[0].as<double>());
       hip_check(hipGetLastError(), "geometry tables");
     }
@@ -1148,9 +1136,7 @@ namespace dgops
     const int64_t nc  = op.mesh.n_cells;
     if (!op.tables_ready[0])
       {
-        op.d_tab_cell[0].alloc(sizeof(double) * dim * op.n_dofs);
-        op.d_cell_det[0].alloc(sizeof(double) * (dim == 3 ? 27 : 9) *
-                               std::max<int64_t>(1, nc));
+        op.d_tab_cell[0].alloc(sizeof(double) * (dim + 1) * op.n_dofs);
         op.d_tab_face[0].alloc(sizeof(double) * dim * nl *
                                std::max<int64_t>(1, nc));
         const OpParams P = op.base_params(0, 1);
@@ -1166,11 +1152,7 @@ namespace dgops
       }
     if (pi == 1)
       {
-        const int64_t ncell = (int64_t)dim * op.n_dofs;
-        const int64_t ndet  = (dim == 3 ? 27 : 9) * nc;
-        op.d_cell_det[1].alloc(sizeof(float) * std::max<int64_t>(1, ndet));
-        vec_copy_convert(op, ndet, DG_FP64, op.d_cell_det[0].ptr, DG_FP32,
-                         op.d_cell_det[1].ptr);
+        const int64_t ncell = (dim + 1) * op.n_dofs;
         const int64_t nface = (int64_t)dim * nl * nc;
         op.d_tab_cell[1].alloc(sizeof(float) * std::max<int64_t>(1, ncell));
         op.d_tab_face[1].alloc(sizeof(float) * std::max<int64_t>(1, nface));

@@ -38,9 +38,6 @@ namespace dgops
     T winv[max_n]; // their reciprocals (0 past n)
     T xq[max_n];   // Gauss points on [0,1]
     T xn[max_n]; // Gauss-Lobatto nodes on [0,1]
-    // quadratic Lagrange basis of the points 0, 1/2, 1 at the Gauss
-    // points: lq[a][i] (det J of a multilinear cell from 3^dim values)
-    T lq[3][max_n];
     EvenOdd<T> eo_S, eo_Sinv, eo_D; // even-odd forms of S, S^-1, D
   };
 
@@ -92,12 +89,6 @@ namespace dgops
         d.winv[i]  = i < b.n ? (T)(1. / b.gauss_wts[i]) : (T)0;
         d.xq[i]    = (T)b.gauss_pts[i];
         d.xn[i]    = (T)b.gl_pts[i];
-        {
-          const double x = b.gauss_pts[i];
-          d.lq[0][i]     = i < b.n ? (T)(2. * (x - 0.5) * (x - 1.)) : (T)0;
-          d.lq[1][i]     = i < b.n ? (T)(-4. * x * (x - 1.)) : (T)0;
-          d.lq[2][i]     = i < b.n ? (T)(2. * x * (x - 0.5)) : (T)0;
-        }
       }
     d.eo_S    = make_even_odd<T>(b.S, b.n);
     d.eo_Sinv = make_even_odd<T>(b.Sinv, b.n);

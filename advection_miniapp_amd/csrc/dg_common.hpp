@@ -48,15 +48,15 @@ namespace dgops
     double        viscosity;
     double        penalty;
     // geometry tables of the tabulated-geometry kernels (scalar type of
-    // the kernel): per cell [dim][nd] = w adj(J) a at the Gauss points,
-    // row d in the line order of direction d (point i of line p at
-    // i * nl + p); per cell [3^dim] = det J at the points {0, 1/2, 1}^dim
-    // (x fastest), from which the kernels interpolate JxW; per
+    // the kernel): per cell [dim + 1][nd] = w adj(J) a (dim rows) and
+    // 1/JxW at the Gauss points, each row in the line order of the
+    // direction it is read along (row d along d, row dim along x: point i
+    // of line p at i * nl + p; JxW of the implicit mass term is recomputed
+    // from the corners along the lines); per
     // cell [dim][nl] = w (adj(J) a)_d on the face at xi_d = 0 (normal
     // along +xi_d); per boundary face [nl][1 + dim] (double) = outward
     // a . n dA and the physical point
     const void   *tab_cell;
-    const void   *cell_det;
     const void   *tab_face;
     const double *tab_bnd;
     const int    *bnd_index; // [cell][2 dim] boundary face id, -1 inside

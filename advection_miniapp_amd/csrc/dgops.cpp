@@ -307,7 +307,6 @@ namespace dgops
       {
         aux_geometry_tables(m, fp32 ? DG_FP32 : DG_FP64);
         Pk.tab_cell = m.d_tab_cell[fp32 ? 1 : 0].ptr;
-        Pk.cell_det = m.d_cell_det[fp32 ? 1 : 0].ptr;
         Pk.tab_face = m.d_tab_face[fp32 ? 1 : 0].ptr;
       }
     else

@@ -111,7 +111,7 @@ namespace dgops
     // tabulated geometry of the operator kernels ([0] fp64, [1] fp32):
     // built on first use, time independent (the velocity's time factor
     // is applied in the kernel)
-    DevBuf d_tab_cell[2], d_cell_det[2], d_tab_face[2];
+    DevBuf d_tab_cell[2], d_tab_face[2];
     bool   tables_ready[2] = {false, false};
     // boundary faces of every geometry mode: index per cell face, a . n dA
     // and x per face point (built once), Dirichlet data at those points

@@ -302,29 +302,10 @@ namespace dgops
   {
     constexpr int nd   = Lay<dim, n>::nd;
     constexpr int nl   = Lay<dim, n>::nl;
-    const T      *base = tab + (cell * dim + row) * nd;
+    const T      *base = tab + (cell * (dim + 1) + row) * nd;
 #pragma unroll
     for (int i = 0; i < n; ++i)
       v[i] = active ? base[i * nl + p] : (T)0;
-  }
-
-  // 1/x for x > 0 and well scaled: the hardware estimate and Newton steps
-  // (relative error ~1 ulp)
-  template <typename T>
-  __device__ __forceinline__ T
-  recip_pos(T x)
-  {
-    if constexpr (sizeof(T) == 8)
-      {
-        double r = __builtin_amdgcn_rcp(x);
-        r        = fma(r, fma(-x, r, 1.), r);
-        return fma(r, fma(-x, r, 1.), r);
-      }
-    else
-      {
-        const float r = __builtin_amdgcn_rcpf(x);
-        return fmaf(r, fmaf(-x, r, 1.f), r);
-      }
   }
 
   // ---- geometry ------------------------------------------------------------
@@ -585,11 +566,6 @@ namespace dgops
     __shared__ T      s_face[dim == 3 ? CPB * FSTR : 1];
     __shared__ T      s_fan[AN ? CPB * 2 * (dim - 1) * nl : 1];
     __shared__ double s_X[CPB * NX];
-    // tabulated geometry with the inverse mass: det J of each cell at the
-    // 3^dim points {0, 1/2, 1}^dim
-    constexpr bool needDet = geo == geo_table && fin == fin_inverse_mass;
-    constexpr int  N3      = dim == 3 ? 27 : 9;
-    __shared__ T   s_det[needDet ? CPB * N3 : 1];
     const T *tab_c = static_cast<const T *>(P.tab_cell);
     const T *tab_f = static_cast<const T *>(P.tab_face);
 
@@ -643,14 +619,6 @@ namespace dgops
       nb[f] = active ? P.neighbors[cell * 2 * dim + f] : -1;
     if constexpr (needX)
       __syncthreads();
-    // (read after the barriers of the value phase)
-    T *det_l = s_det + (needDet ? lcs * N3 : 0);
-    if constexpr (needDet)
-      {
-        if (active)
-          for (int e = p; e < N3; e += nl)
-            det_l[e] = static_cast<const T *>(P.cell_det)[cell * N3 + e];
-      }
 
     // tabulated geometry: LAST-direction speed on this thread's line, the
     // y and x speed rows, and a . n dA on the faces its lines end on (lower
@@ -1218,35 +1186,7 @@ namespace dgops
                   ij[i] = B.winv[i] * r;
               }
             else
-              {
-                // 1/JxW on this thread's x-line: det J of the multilinear
-                // cell is of degree 2 per direction, so its 3^dim values
-                // give it exactly at the Gauss points
-                int q[3];
-                L::template idx<0>(p, 0, q);
-                T d2[3] = {0, 0, 0};
-#pragma unroll
-                for (int c = 0; c < (dim == 3 ? 3 : 1); ++c)
-#pragma unroll
-                  for (int b = 0; b < 3; ++b)
-                    {
-                      const T lbc =
-                        dim == 3 ? B.lq[b][q[1]] * B.lq[c][q[2]] : B.lq[b][q[1]];
-#pragma unroll
-                      for (int a = 0; a < 3; ++a)
-                        d2[a] = fma(det_l[(c * 3 + b) * 3 + a], lbc, d2[a]);
-                    }
-                const T wt =
-                  B.winv[q[1]] * (dim == 3 ? B.winv[q[2]] : (T)1);
-#pragma unroll
-                for (int i = 0; i < n; ++i)
-                  {
-                    const T det = fma(d2[2], B.lq[2][i],
-                                      fma(d2[1], B.lq[1][i],
-                                          d2[0] * B.lq[0][i]));
-                    ij[i] = active ? B.winv[i] * wt * recip_pos(det) : (T)0;
-                  }
-              }
+              table_line<dim, n>(tab_c, cell, dim, p, active, ij);
 #pragma unroll
             for (int i = 0; i < n; ++i)
               acc[i] *= ij[i];
