@@ -75,6 +75,11 @@ CASES = {
                               velocity_kind=3, time_factor=1, final_time=8.0,
                               factor_skew=0.5),
 }
+SEPARABLE = {
+    "3d_k4_cart": CASES["3d_k4_cart"],
+    "2d_k5_cart": dict(dim=2, degree=5, refine=2, velocity=[-0.7, 1.1],
+                       factor_skew=0.5),
+}
 
 
 @pytest.mark.parametrize("s", [2, 3])
@@ -107,14 +112,15 @@ def test_cell_operator_matches_oracle_on_a_checkerboard(gpu_lib, name, s):
         assert err < 1e-12, (side, err)
 
 
-def test_cell_solver_is_exact_for_a_separable_cell_operator(gpu_lib):
+@pytest.mark.parametrize("name", sorted(SEPARABLE))
+def test_cell_solver_is_exact_for_a_separable_cell_operator(gpu_lib, name):
     """Axis-aligned cells and a constant speed: the cell operator is the
     separable one the collocation FDM inverts, so the GMRES step has
     alpha = 1 and the preconditioner solves every cell exactly. In the
     collocation basis A_c = S^-T A S^-1: the correction x satisfies
     A_c (S x) = S^-T r, and equals the nodal FDM's exact cell inverse."""
     s = 3
-    c = Cfg(irk_stages=s, **CASES["3d_k4_cart"])
+    c = Cfg(irk_stages=s, **SEPARABLE[name])
     op = AdvectionOperation(c.dg())
     n, dim = c.degree + 1, c.dim
     S = basis_change(c.degree)
