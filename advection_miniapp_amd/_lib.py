@@ -129,6 +129,8 @@ SIGNATURES = {
     "dg_irk_cell_operator": (_i, [_vp, _vp, _vp]),
     "dg_irk_coefficients": (_i, [_i, _dp, _dp, _dp]),
     "dg_irk_solve": (_i, [_vp, _vp, _vp, ctypes.POINTER(DgSolverControl)]),
+    "dg_irk_block_jacobi_vmult": (_i, [_vp, ctypes.POINTER(DgSolverControl),
+                                       _vp, _vp]),
     "dg_irk_time_step": (_i, [_vp, _d, _d, _vp,
                               ctypes.POINTER(DgSolverControl)]),
     "dg_dirk_time_step": (_i, [_vp, _i, _d, _d, _vp,

@@ -680,7 +680,8 @@ namespace dgops
     {
       const unsigned nb = std::min<unsigned>(grid_for(n), 1024);
       op.d_partials.alloc(sizeof(double) * nb * max_multi);
-      op.d_result.alloc(sizeof(double) * 64);
+      // one sum per vector x_j: room for all m of them
+      op.d_result.alloc(sizeof(double) * std::max(64, m));
       double *part = op.d_partials.as<double>();
       int     done = 0;
       std::vector<double> host(m);

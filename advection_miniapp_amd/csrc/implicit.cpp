@@ -905,6 +905,86 @@ dg_irk_cell_operator(dg_operator *h, void *dst, const void *src)
   });
 }
 
+} // extern "C"
+
+namespace
+{
+  // The preconditioner the outer (F)GMRES of the IRK solve applies, as
+  // selected by a solver control: the FDM block-Jacobi step (or the batched
+  // cell solver) in the control's precision, or an inner GMRES of a fixed
+  // number of steps around it (BlockJacobi::vmult,
+  // advection_solver_irk_real.cc:2462-2479, with the build's precision and
+  // inner-solver choices). Vectors are FP64 block vectors.
+  std::function<void(const void *, void *)>
+  irk_outer_preconditioner(Operator &op, const dg_solver_control &ctl_in)
+  {
+    require(ctl_in.inner_iterations >= 0 && ctl_in.inner_iterations <= 16,
+            "inner GMRES: 0..16 iterations", DG_ERR_UNSUPPORTED);
+    setup_irk(op);
+    const int     s     = op.irk.s;
+    const int64_t N     = op.n_dofs * s;
+    const double  t     = op.time, dt = op.time_step;
+    const double  vs    = avg_stage_factor(op, t, dt);
+    const int     pp    = ctl_in.prec_precision;
+    const int     inner = ctl_in.inner_iterations;
+    const bool    cs    = ctl_in.cell_solver != 0;
+    Operator     *o     = &op;
+    if (inner > 0)
+      {
+        // inner right-preconditioned GMRES in `pp` with the FDM (or the
+        // batched cell solver), a fixed number of steps on the device
+        require(ctl_in.flexible, "an inner solver needs the flexible method");
+        const bool f32 = pp == DG_FP32;
+        return [=](const void *src, void *dst) {
+          LinOp in;
+          in.prec = f32 ? DG_FP32 : DG_FP64;
+          in.n    = N;
+          in.A    = [=](const void *x, void *y) {
+            vmult_irk(*o, f32, t, dt, x, y);
+          };
+          in.P = [=](const void *x, void *y) {
+            if (cs)
+              cell_gmres_precondition(*o, f32, t, dt, vs, x, y);
+            else
+              launch_fdm_variant(*o, s, f32 ? 1 : 0, vs, 1. / dt, x, y);
+          };
+          gmres_inner_device(*o, in, dst, src, inner, 20);
+        };
+      }
+    const int prec = pp == DG_FP32 ? 2 : 0;
+    require(!cs || prec == 0,
+            "the batched cell solver runs in the precision of its "
+            "vectors: FP32 through an inner solver (inner_iterations)",
+            DG_ERR_UNSUPPORTED);
+    return [=](const void *x, void *y) {
+      if (cs)
+        cell_gmres_precondition(*o, false, t, dt, vs, x, y);
+      else
+        launch_fdm_variant(*o, s, prec, vs, 1. / dt, x, y);
+    };
+  }
+} // namespace
+
+extern "C" {
+
+int
+dg_irk_block_jacobi_vmult(dg_operator *h, const dg_solver_control *ctl,
+                          void *dst, const void *src)
+{
+  return guard([&] {
+    Operator *op = get(h);
+    require(dst && src && ctl && dst != src,
+            "distinct non-null vectors and a control");
+    require(!op->fp32(), "the preconditioner of the FP64 outer solver "
+                         "takes FP64 vectors: create the operator with "
+                         "precision DG_FP64",
+            DG_ERR_STATE);
+    require(op->time_step != 0., "preconditioner: set_time with dt != 0",
+            DG_ERR_STATE);
+    irk_outer_preconditioner(*op, *ctl)(src, dst);
+  });
+}
+
 int
 dg_irk_solve(dg_operator *h, void *stage, const void *rhs,
              dg_solver_control *ctl)
@@ -917,56 +997,17 @@ dg_irk_solve(dg_operator *h, void *stage, const void *rhs,
             DG_ERR_STATE);
     require(op->time_step != 0., "irk solve: set_time with dt != 0",
             DG_ERR_STATE);
-    require(ctl->inner_iterations >= 0 && ctl->inner_iterations <= 16,
-            "inner GMRES: 0..16 iterations", DG_ERR_UNSUPPORTED);
     setup_irk(*op);
     const int     s  = op->irk.s;
     const int64_t N  = op->n_dofs * s;
     const double  t  = op->time, dt = op->time_step;
-    const double  vs = avg_stage_factor(*op, t, dt);
     LinOp outer;
     outer.prec = DG_FP64;
     outer.n    = N;
     outer.A    = [&](const void *x, void *y) {
       vmult_irk(*op, false, t, dt, x, y);
     };
-    const int pp = ctl->prec_precision;
-    if (ctl->inner_iterations > 0)
-      {
-        // inner right-preconditioned GMRES in `pp` with the FDM (or the
-        // batched cell solver), a fixed number of steps on the device
-        require(ctl->flexible, "an inner solver needs the flexible method");
-        const bool f32 = pp == DG_FP32;
-        outer.P = [&, f32](const void *src, void *dst) {
-          LinOp in;
-          in.prec = f32 ? DG_FP32 : DG_FP64;
-          in.n    = N;
-          in.A    = [&, f32](const void *x, void *y) {
-            vmult_irk(*op, f32, t, dt, x, y);
-          };
-          in.P = [&, f32](const void *x, void *y) {
-            if (ctl->cell_solver)
-              cell_gmres_precondition(*op, f32, t, dt, vs, x, y);
-            else
-              launch_fdm_variant(*op, s, f32 ? 1 : 0, vs, 1. / dt, x, y);
-          };
-          gmres_inner_device(*op, in, dst, src, ctl->inner_iterations, 20);
-        };
-      }
-    else
-      {
-        const int prec = pp == DG_FP32 ? 2 : 0;
-        require(!ctl->cell_solver || prec == 0,
-                "the batched cell solver runs in the precision of its "
-                "vectors: FP32 through an inner solver (inner_iterations)",
-                DG_ERR_UNSUPPORTED);
-        outer.P = [&, prec](const void *x, void *y) {
-          if (ctl->cell_solver)
-            cell_gmres_precondition(*op, false, t, dt, vs, x, y);
-          else
-            launch_fdm_variant(*op, s, prec, vs, 1. / dt, x, y);
-        };
-      }
+    outer.P = irk_outer_preconditioner(*op, *ctl);
     double bn2;
     vec_multi_dot(*op, DG_FP64, N, 1, rhs, N, rhs, &bn2);
     const double tol = std::max(ctl->abs_tolerance,

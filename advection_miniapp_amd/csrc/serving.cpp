@@ -10,6 +10,7 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "aux.hpp"
@@ -114,6 +115,8 @@ namespace dgops
       DevBuf           reg[3];
       int64_t          n_cells, n_dofs;
       int64_t          op_bytes, vec_bytes;
+      int              err_code = DG_OK; // setup error of a FAILED request
+      std::string      err_msg;
 
       Operator *op() const { return reinterpret_cast<Operator *>(handle); }
       ~ServeReq()
@@ -156,6 +159,22 @@ namespace dgops
                              sizeof(DevBasis<float>));
     }
 
+    // leave no half-built request behind: the caller sees the error of
+    // the admitting call, the request's FAILED state, and the same error
+    // again when it takes (and so releases) the request
+    static void
+    fail(ServeReq &q, int code, const std::string &msg)
+    {
+      for (auto &r : q.reg)
+        r.release();
+      if (q.handle)
+        dg_destroy(q.handle);
+      q.handle   = nullptr;
+      q.state    = DG_REQUEST_FAILED;
+      q.err_code = code;
+      q.err_msg  = "request " + std::to_string(q.id) + " setup: " + msg;
+    }
+
     void
     admit()
     {
@@ -177,16 +196,14 @@ namespace dgops
                 r.alloc(q.vec_bytes);
               aux_project(*q.op(), q.reg[0].ptr, 0.);
             }
-          catch (...)
+          catch (const Error &e)
             {
-              // leave no half-built request behind; the caller sees the
-              // error of this call and the request's FAILED state
-              for (auto &r : q.reg)
-                r.release();
-              if (q.handle)
-                dg_destroy(q.handle);
-              q.handle = nullptr;
-              q.state  = DG_REQUEST_FAILED;
+              fail(q, e.code, e.what());
+              throw;
+            }
+          catch (const std::exception &e)
+            {
+              fail(q, DG_ERR_ARGUMENT, e.what());
               throw;
             }
           q.state = DG_REQUEST_ACTIVE;
@@ -413,6 +430,13 @@ dg_server_take_result(dg_server *hs, int64_t id, void *dst)
     auto    it = S.reqs.find(id);
     require(it != S.reqs.end(), "unknown request");
     ServeReq &q = *it->second;
+    if (q.state == DG_REQUEST_FAILED)
+      {
+        // release it and report its setup error
+        const Error e(q.err_code, q.err_msg);
+        S.reqs.erase(it);
+        throw e;
+      }
     require(q.state == DG_REQUEST_DONE, "request not finished",
             DG_ERR_STATE);
     hip_check(hipMemcpyAsync(dst, q.reg[0].ptr, q.vec_bytes,

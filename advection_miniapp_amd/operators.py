@@ -211,6 +211,15 @@ class AdvectionOperation:
         L.check(self._lib.dg_irk_cell_operator(
             self._h, self._v(dst, "dst", s), self._v(src, "src", s)))
 
+    def block_jacobi_vmult(self, dst, src, control):
+        """The preconditioner irk_solve applies for `control` (FDM in the
+        control's precision on FP64 block vectors, cell solver or inner
+        GMRES): BlockJacobi::vmult, advection_solver_irk_real.cc:2462-2479."""
+        s = self.n_stages
+        L.check(self._lib.dg_irk_block_jacobi_vmult(
+            self._h, ctypes.byref(control), self._v(dst, "dst", s),
+            self._v(src, "src", s)))
+
     def irk_solve(self, stage, rhs, control):
         """(F)GMRES on the stage system from the initial guess in `stage`
         (SolverFGMRES::solve, advection_solver_irk_real.cc:2729-2757);

@@ -15,10 +15,18 @@ IRK workload: 32^3 cells on [0,1]^3, degree 4, the same velocity field,
 Radau IIA with 3 stages (12,288,000 unknowns), dt = 5 h_min / 2 (Courant
 number 5, advection_solver_irk_real.cc:2352), flexible GMRES(30) in FP64 to
 a 1e-10 relative residual from a zero initial guess, cellwise FDM
-preconditioner in FP64 against the FP32 variants (FP32 FDM, and FP32 inner
-GMRES of 4 or 8 steps; the fastest is reported as the mixed method).
+preconditioner: three algorithms (FDM alone, inner GMRES of 4 or 8 steps
+around the FDM), each in FP64 and with the preconditioner in FP32; the
+speedup is the fastest all-FP64 solve over the fastest mixed one, with the
+same-algorithm ratios beside it.
+
+Also by default: configuration 1 (2D run, L2 error), configuration 3
+(SIPG, degrees 3/5/7, FP64 and FP32), the Burgers add-on of configuration 4,
+the serving stream (configuration 5) against serial solves, and the CPU
+baseline (the oracle on the host cores).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-irk]
+                    [--no-configs] [--no-serving] [--extra]
 
 Prints one JSON line.
 """
@@ -52,31 +60,42 @@ def irk_workload(refine=5):
                 periodic=[1, 1, 1], lo=[0, 0, 0], hi=[1, 1, 1])
 
 
-def cpu_baseline(threads=None, target_seconds=12.0):
-    """The oracle's apply (restated reference algorithm, OpenMP over cells)
-    on a bounded sample: 16^3 cells of the apply workload."""
-    sys.path.insert(0, ROOT)
+def _oracle_apply_rate(threads, refine, target_seconds, min_reps):
     import oracle
     from tests.helpers import Cfg
-    cores = threads or len(os.sched_getaffinity(0))
-    cores = min(cores, 16)
-    oracle.load().or_set_threads(cores)
-    o = Cfg(**apply_workload(refine=4)).oracle_handle()
+    oracle.load().or_set_threads(threads)
+    o = Cfg(**apply_workload(refine=refine)).oracle_handle()
     u = o.project(0.0)
-    o.apply(u, 0.0)  # warm up
     reps, t0 = 0, time.perf_counter()
     while True:
         o.apply(u, 0.0)
         reps += 1
         el = time.perf_counter() - t0
-        if el > target_seconds or reps >= 2000:
+        if (el > target_seconds and reps >= min_reps) or reps >= 2000:
             break
-    return dict(value=o.n_dofs * reps / el, unit="DoF/s", cores=cores,
-                kind="port",
-                sample=f"{reps} applies on 16^3 cells (degree 4, "
-                       f"{o.n_dofs} DoF) of the same workload in {el:.1f} s, "
+    n = o.n_dofs
+    del o, u
+    return n * reps / el, reps, el, n
+
+
+def cpu_baseline(target_seconds=10.0):
+    """The oracle's apply (restated reference algorithm, OpenMP over cells)
+    on the host cores of this GPU's box: all of this process's cores, at
+    most the 16 a one-GPU box allots, on the full apply workload (64^3
+    cells); and one core on a 32^3-cell sample of the same workload."""
+    sys.path.insert(0, ROOT)
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    v_all, reps, el, n = _oracle_apply_rate(cores, 6, target_seconds, 2)
+    v_one, reps1, el1, n1 = _oracle_apply_rate(1, 5, target_seconds, 1)
+    return dict(value=v_all, unit="DoF/s", cores=cores, kind="port",
+                sample=f"{reps} applies of the full workload (64^3 cells, "
+                       f"degree 4, {n} DoF) in {el:.1f} s on {cores} cores; "
                        "restated reference algorithm (oracle/oracle.cpp, "
-                       "OpenMP), not deal.II")
+                       "OpenMP over cells), not deal.II",
+                one_core=dict(value=v_one, unit="DoF/s", cores=1,
+                              sample=f"{reps1} applies on 32^3 cells of the "
+                                     f"same workload ({n1} DoF) in "
+                                     f"{el1:.1f} s"))
 
 
 def measured_traffic():
@@ -131,10 +150,15 @@ def time_irk(args):
     rhs = op.initialize_dof_vector(blocks=s)
     op.compute_rhs(rhs, u)
     rn = rhs.norm().item()
+    # each algorithm (FDM-preconditioned FGMRES, and FGMRES around an inner
+    # GMRES of 4 or 8 steps with the FDM) in FP64 and with the
+    # preconditioner (and the inner solver) in FP32
     variants = {
         "fp64": dict(prec_precision=L.DG_FP64),
         "fp32_fdm": dict(prec_precision=L.DG_FP32),
+        "fp64_inner4": dict(prec_precision=L.DG_FP64, inner_iterations=4),
         "fp32_inner4": dict(prec_precision=L.DG_FP32, inner_iterations=4),
+        "fp64_inner8": dict(prec_precision=L.DG_FP64, inner_iterations=8),
         "fp32_inner8": dict(prec_precision=L.DG_FP32, inner_iterations=8),
     }
     out, sol = {}, {}
@@ -162,13 +186,23 @@ def time_irk(args):
     for name in variants:
         out[name]["rel_diff_vs_fp64"] = \
             ((sol[name] - ref).norm() / ref.norm()).item()
-    best = min((n for n in variants if n != "fp64"),
-               key=lambda n: out[n]["ms"])
+    # headline: the fastest all-FP64 solve against the fastest solve with
+    # an FP32 preconditioner; beside it the ratio of each algorithm run in
+    # both precisions
+    best64 = min((n for n in variants if n.startswith("fp64")),
+                  key=lambda n: out[n]["ms"])
+    best32 = min((n for n in variants if n.startswith("fp32")),
+                 key=lambda n: out[n]["ms"])
+    same = {alg: out[a]["ms"] / out[b]["ms"] for alg, a, b in
+            (("fdm", "fp64", "fp32_fdm"),
+             ("inner4", "fp64_inner4", "fp32_inner4"),
+             ("inner8", "fp64_inner8", "fp32_inner8"))}
     return dict(workload=f"3D IRK Radau IIA(3), degree 4, "
                          f"{2 ** args.irk_refine}^3 cells, Courant 5, "
                          "FGMRES(30) FP64 to 1e-10 from zero",
-                unknowns=op.n_dofs * s, mixed=best,
-                speedup=out["fp64"]["ms"] / out[best]["ms"], **out)
+                unknowns=op.n_dofs * s, fp64_best=best64, mixed=best32,
+                speedup=out[best64]["ms"] / out[best32]["ms"],
+                speedup_same_algorithm=same, **out)
 
 
 def serving_requests(n_req, final_time, seed=2024, rate=200.0):
@@ -508,14 +542,15 @@ def time_implicit_parts(steps=20):
     return out
 
 
-def time_extra():
-    """Configuration 3 (advection + SIPG diffusion, 32^3 cells, degrees
-    3/5/7, FP64 and FP32 operators, FP32 error against FP64) and the
-    Burgers add-on of configuration 4 (degree 3, 48^3 cells on [-1,1]^3,
-    u0 = (-sin pi x, 0, 0))."""
+def time_config3():
+    """Configuration 3: advection + SIPG diffusion, 32^3 cells, degrees
+    3/5/7, a = (1, 0.5, 0.25), nu = 1e-3, penalty (k+1)^2, product of
+    sines; FP64 and FP32 operator applies and the FP32 result's error
+    against the FP64 one (the FP64 apply equals the oracle to 1e-12 in the
+    parity tests)."""
     import math
     import torch
-    out = {"sipg": {}, "burgers": {}}
+    out = {}
     for k in (3, 5, 7):
         base = dict(dim=3, degree=k, refine=5, velocity=[1.0, 0.5, 0.25],
                     operator_kind=1, viscosity=1e-3,
@@ -528,26 +563,41 @@ def time_extra():
                                            src=src.float())
         err = ((dst32.double() - dst64).abs().max() /
                dst64.abs().max()).item()
-        out["sipg"][f"k{k}"] = dict(
+        out[f"k{k}"] = dict(
             n_dofs=n, fp64_ms=ms64, fp64_dof_per_s=n / (ms64 * 1e-3),
             fp32_ms=ms32, fp32_dof_per_s=n / (ms32 * 1e-3),
             fp32_rel_err_vs_fp64=err)
         del src, dst64, dst32
         torch.cuda.empty_cache()
-    out["apply_by_degree"] = time_apply_by_degree()
-    out["config2_run"] = time_config2_run()
-    out["inverse_mass"] = time_inverse_mass()
-    out["config1_run"] = time_config1_run()
-    out["cell_solver"] = time_cell_solver()
-    out["implicit"] = time_implicit_parts()
+    return out
+
+
+def time_burgers():
+    """The Burgers add-on of configuration 4: degree 3, 48^3 cells on
+    [-1,1]^3 (3 x 3 x 3 coarse cells, 4 refinements), 3 components,
+    u0 = (-sin pi x, 0, 0), over-integration QGauss(6): one right-hand
+    side per step."""
     bk = dict(dim=3, degree=3, n_components=3, operator_kind=2,
               n_coarse=[3, 3, 3], refine=4, lo=[-1, -1, -1], hi=[1, 1, 1],
               solution_kind=6)
     op, _, _, ms = time_one_apply(bk)
-    out["burgers"] = dict(workload="3D Burgers, degree 3, 48^3 cells, "
-                                   "3 components, over-integration 6^3",
-                          unknowns=op.vector_size, ms=ms,
-                          unknowns_per_s=op.vector_size / (ms * 1e-3))
+    return dict(workload="3D Burgers, degree 3, 48^3 cells, 3 components, "
+                         "over-integration 6^3",
+                unknowns=op.vector_size, ms=ms,
+                unknowns_per_s=op.vector_size / (ms * 1e-3),
+                gb_per_s=16 * op.vector_size / (ms * 1e-3) / 1e9)
+
+
+def time_extra():
+    """Further measurements: the configuration-2 apply by degree, the
+    configuration-2 run, the standalone inverse mass, the batched cell
+    solver and the implicit operator parts."""
+    out = {}
+    out["apply_by_degree"] = time_apply_by_degree()
+    out["config2_run"] = time_config2_run()
+    out["inverse_mass"] = time_inverse_mass()
+    out["cell_solver"] = time_cell_solver()
+    out["implicit"] = time_implicit_parts()
     return out
 
 
@@ -561,11 +611,14 @@ def main():
     ap.add_argument("--irk-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-irk", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the configuration 1, 3 and Burgers legs")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the serving workload (about 20 s)")
     ap.add_argument("--extra", action="store_true",
-                    help="also time the SIPG (k = 3, 5, 7) and Burgers "
-                         "operators")
-    ap.add_argument("--serving", action="store_true",
-                    help="also run the serving workload (tens of seconds)")
+                    help="also time the apply by degree, the configuration-2 "
+                         "run, the inverse mass, the cell solver and the "
+                         "implicit operator parts")
     ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--serve-final-time", type=float, default=0.5)
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
@@ -621,7 +674,11 @@ def main():
     del op
     if not args.no_irk:
         out["irk"] = time_irk(args)
-    if args.serving:
+    if not args.no_configs:
+        out["config1"] = time_config1_run()
+        out["config3_sipg"] = time_config3()
+        out["config4_burgers"] = time_burgers()
+    if not args.no_serving:
         out["serving"] = time_serving(args)
     if args.extra:
         out["extra"] = time_extra()

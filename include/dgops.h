@@ -273,6 +273,16 @@ int dg_irk_cell_operator(dg_operator *op, void *dst, const void *src);
 int dg_irk_solve(dg_operator *op, void *stage_blocks, const void *rhs_blocks,
                  dg_solver_control *control);
 
+/* The preconditioner that dg_irk_solve applies for `control` (FP64 block
+ * vectors): the cellwise FDM block-Jacobi step in control->prec_precision
+ * (FP32 arithmetic on the FP64 vectors for DG_FP32), the batched cell
+ * solver (cell_solver), or an inner GMRES of inner_iterations steps around
+ * either; the outputs of `control` are not touched.
+ *                          BlockJacobi::vmult advection_solver_irk_real.cc:2462-2479 */
+int dg_irk_block_jacobi_vmult(dg_operator *op,
+                              const dg_solver_control *control, void *dst,
+                              const void *src);
+
 /* One Radau IIA step u_{n+1} = u_n + dt b^T A^-1 Z, with a zero initial
  * guess.                               run() loop :2533-2779 without projection */
 int dg_irk_time_step(dg_operator *op, double current_time, double time_step,
@@ -382,7 +392,8 @@ int dg_server_request_info(dg_server *s, int64_t id, int *state,
                            int64_t *n_dofs, int *steps_done,
                            int *steps_total);
 /* copy a finished request's solution (n_dofs doubles, device pointer dst)
- * and release it                                                         */
+ * and release it; a FAILED request is released without a copy and the
+ * call returns the error of its setup                                     */
 int dg_server_take_result(dg_server *s, int64_t id, void *dst);
 /* device memory held by admitted and finished requests                  */
 int dg_server_memory(dg_server *s, int64_t *bytes_in_use);

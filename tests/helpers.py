@@ -93,3 +93,42 @@ def nodal_points(verts, n, dim):
             b = (v >> d) & 1
             N[:, v] *= ref[:, d] if b else 1 - ref[:, d]
     return np.einsum("pv,cvd->cpd", N, verts)
+
+
+def cellular_velocity(p):
+    """(sin2πy cos2πz, sin2πz cos2πx, sin2πx cos2πy) at points [..., 3]."""
+    t = 2 * np.pi
+    x, y, z = p[..., 0], p[..., 1], p[..., 2]
+    return np.stack([np.sin(t * y) * np.cos(t * z),
+                     np.sin(t * z) * np.cos(t * x),
+                     np.sin(t * x) * np.cos(t * y)], axis=-1)
+
+
+def sine_product_gradient(p, w=2 * np.pi):
+    """Gradient of prod_d sin(w x_d) at points [..., 3]."""
+    S, C = np.sin(w * p), np.cos(w * p)
+    return w * np.stack([C[..., 0] * S[..., 1] * S[..., 2],
+                         S[..., 0] * C[..., 1] * S[..., 2],
+                         S[..., 0] * S[..., 1] * C[..., 2]], axis=-1)
+
+
+def transport_error(du, verts, k, velocity_kind, velocity=(0.7, -0.4, 0.3)):
+    """Relative RMS difference at the nodes between an explicit right-hand
+    side du = M^-1(-L u_h) of u = prod sin(2 pi x_d) (3D) and the exact
+    -a.grad u, for the cellular (velocity_kind 1) or a constant speed."""
+    p = nodal_points(verts, k + 1, 3)
+    if velocity_kind == 1:
+        a = cellular_velocity(p)
+    else:
+        a = np.broadcast_to(np.asarray(velocity, dtype=float), p.shape)
+    ex = -(a * sine_product_gradient(p)).sum(-1).ravel()
+    du = np.asarray(du, dtype=np.float64)
+    return float(np.sqrt(np.mean((du - ex) ** 2)) / np.sqrt(np.mean(ex ** 2)))
+
+
+def transport_problem(k, refine, velocity_kind, perturbation=0.1, **kw):
+    """3D periodic unit cube, perturbed interior vertices, u = prod
+    sin(2 pi x_d): the problem of the geometry convergence tests."""
+    return Cfg(dim=3, degree=k, refine=refine, perturbation=perturbation,
+               velocity_kind=velocity_kind, velocity=[0.7, -0.4, 0.3],
+               solution_kind=2, solution_param=[2 * np.pi, 0.0], **kw)

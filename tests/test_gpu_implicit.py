@@ -166,55 +166,169 @@ def dense_cell_operator(M, Ks, abar, det, sigma, dim):
     return det * A
 
 
+def fdm_fp32_tol(k, dim):
+    """Bound of the FP32 FDM against the dense FP64 inverse: the eigenvector
+    matrix V of the upwind 1D operator M^-1 K is not orthogonal, and FP32
+    rounding in the dim sweeps each way (and in the pointwise denominators)
+    is amplified by up to cond(V)^dim: eps32 cond(V)^dim, with V from
+    numpy's eigendecomposition of the 1D matrices built here (cond V = 25,
+    90, 321, 1150 at k = 3..6). A wrong eigenpair, transform or speed gives
+    O(1) errors."""
+    M, Ks = one_d_matrices(k, 1.0)
+    V = np.linalg.eig(np.linalg.solve(M, Ks[0]))[1]
+    return np.finfo(np.float32).eps * np.linalg.cond(V) ** dim
+
+
 @pytest.mark.parametrize("dim,k,vel", [(3, 4, [1.0, -0.5, 0.25]),
                                        (3, 3, [-0.3, 0.7, -1.2]),
                                        (2, 5, [0.4, -1.0, 0]),
                                        (2, 6, [-0.4, 1.0, 0])])
-def test_fdm_is_inverse_of_separable_operator(gpu_lib, dim, k, vel):
+@pytest.mark.parametrize("prec", [0, 1])
+def test_fdm_is_inverse_of_separable_operator(gpu_lib, dim, k, vel, prec):
+    """Single-stage FDM (dg_precondition_block_jacobi) in FP64 and, with an
+    FP32 operator, on FP32 vectors in FP32 arithmetic."""
     h = 0.25
-    c = Cfg(dim=dim, degree=k, refine=2, velocity=vel, factor_skew=0.5)
+    c = Cfg(dim=dim, degree=k, refine=2, velocity=vel, factor_skew=0.5,
+            precision=prec)
     op = AdvectionOperation(c.dg())
     nd = (k + 1) ** dim
     r = np.random.default_rng(7).standard_normal(op.n_dofs)
     dt = 0.03
     op.set_time(0.0, dt)
     z = op.initialize_dof_vector()
-    op.precondition_block_jacobi(z, dev(r))
+    op.precondition_block_jacobi(z, dev(r, op.dtype))
     M, Ks = one_d_matrices(k, 1.0)
     abar = [v / h for v in vel[:dim]]
     A = dense_cell_operator(M, Ks, abar, h ** dim, 1 / dt, dim)
     zz = host(z).reshape(-1, nd)
     rr = r.reshape(-1, nd)
+    tol = 1e-10 if prec == 0 else fdm_fp32_tol(k, dim)
     for cell in (0, 5, op.n_cells - 1):
-        assert rel_diff(zz[cell], np.linalg.solve(A, rr[cell])) < 1e-10
+        err = rel_diff(zz[cell], np.linalg.solve(A, rr[cell]))
+        print(f"fdm dim {dim} k {k} prec {prec} cell {cell}: {err:.2e}")
+        assert err < tol
 
 
+def irk_fdm_apply(op, z, r, prec):
+    """FDM of the IRK stage system: prec 0 FP64, 1 an FP32 operator on FP32
+    vectors, 2 FP32 arithmetic on FP64 vectors (the block-Jacobi step of the
+    mixed-precision FGMRES, through dg_irk_block_jacobi_vmult)."""
+    if prec == 2:
+        ctl = control(prec_precision=L.DG_FP32)
+        op.block_jacobi_vmult(z, r, ctl)
+    else:
+        op.precondition_block_jacobi_irk(z, r)
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2])
 @pytest.mark.parametrize("s", [2, 3, 4])
-def test_irk_fdm_is_inverse_of_stage_operator(gpu_lib, s):
+def test_irk_fdm_is_inverse_of_stage_operator(gpu_lib, s, prec):
     k, h, dim = 3, 0.25, 3
     vel = [0.6, -1.0, 0.3]
     c = Cfg(dim=dim, degree=k, refine=2, velocity=vel, factor_skew=0.5,
-            irk_stages=s)
+            irk_stages=s, precision=1 if prec == 1 else 0)
     op = AdvectionOperation(c.dg())
     nd = (k + 1) ** dim
     N = op.n_dofs
     r = np.random.default_rng(8).standard_normal(N * s)
     dt = 0.07
     op.set_time(0.0, dt)
-    for prec, tol in ((None, 1e-10),):
-        z = op.initialize_dof_vector(blocks=s)
-        op.precondition_block_jacobi_irk(z, dev(r))
-        A, b, cc = irk_coefficients(s)
-        M, Ks = one_d_matrices(k, 1.0)
-        abar = [v / h for v in vel]
-        Lsep = dense_cell_operator(M, Ks, abar, h ** dim, 0.0, dim)
-        Mc = dense_cell_operator(M, Ks, [0, 0, 0], h ** dim, 1.0, dim)
+    z = op.initialize_dof_vector(blocks=s)
+    irk_fdm_apply(op, z, dev(r, op.dtype), prec)
+    A, b, cc = irk_coefficients(s)
+    M, Ks = one_d_matrices(k, 1.0)
+    abar = [v / h for v in vel]
+    Lsep = dense_cell_operator(M, Ks, abar, h ** dim, 0.0, dim)
+    Mc = dense_cell_operator(M, Ks, [0, 0, 0], h ** dim, 1.0, dim)
+    Big = np.kron(np.linalg.inv(A) / dt, Mc) + np.kron(np.eye(s), Lsep)
+    zz = host(z).reshape(s, -1, nd)
+    rr = r.reshape(s, -1, nd)
+    tol = 1e-10 if prec == 0 else fdm_fp32_tol(k, dim)
+    for cell in (0, 17, op.n_cells - 1):
+        ref = np.linalg.solve(Big, rr[:, cell, :].ravel())
+        err = rel_diff(zz[:, cell, :].ravel(), ref)
+        print(f"irk fdm s {s} prec {prec} cell {cell}: {err:.2e}")
+        assert err < tol
+
+
+def cell_fdm_data(verts, k, dim, velocity):
+    """Per cell, from the corners alone: the JxW-weighted mean of the
+    reference-space speed J^-1 a over the QGauss(k+1) points and det J at
+    the first quadrature point -- the data of the reference's FDM reinit
+    (scaled_cell_velocity, advection_solver_irk_real.cc:1543-1556, and
+    determinant(inverse_jacobian(0)), :2101-2106), restated with numpy."""
+    n = k + 1
+    xq, wq = np.polynomial.legendre.leggauss(n)
+    xq, wq = (xq + 1) / 2, wq / 2
+    nv = 2 ** dim
+    abar = np.zeros((len(verts), dim))
+    det0 = np.zeros(len(verts))
+    for cell, X in enumerate(verts):
+        num, vol, first = np.zeros(dim), 0.0, True
+        for idx in np.ndindex(*(n,) * dim):
+            q = idx[::-1]                  # lexicographic, x fastest
+            xi = np.array([xq[q[d]] for d in range(dim)])
+            w = np.prod([wq[q[d]] for d in range(dim)])
+            x = np.zeros(dim)
+            J = np.zeros((dim, dim))
+            for v in range(nv):
+                bits = [(v >> d) & 1 for d in range(dim)]
+                N = np.prod([xi[d] if bits[d] else 1 - xi[d]
+                             for d in range(dim)])
+                x += N * X[v]
+                for d in range(dim):
+                    dN = np.prod([(1 if bits[e] else -1) if e == d else
+                                  (xi[e] if bits[e] else 1 - xi[e])
+                                  for e in range(dim)])
+                    J[:, d] += dN * X[v]
+            a = velocity(x)
+            detJ = np.linalg.det(J)
+            num += np.linalg.solve(J, a) * detJ * w
+            vol += detJ * w
+            if first:
+                det0[cell], first = detJ, False
+        abar[cell] = num / vol
+    return abar, det0
+
+
+@pytest.mark.parametrize("prec", [0, 2])
+def test_irk_fdm_on_perturbed_cells(gpu_lib, prec):
+    """Perturbed cells and the cellular speed: the FDM inverts the
+    separable operator det J(q0) (A^-1/dt (x) M (x) M (x) M + sum_d abar_d
+    M (x) .. K_d .. (x) M) with abar and det J(q0) recomputed here from the
+    exported corners (cell_fdm_data)."""
+    k, dim, s = 3, 3, 3
+    c = Cfg(dim=dim, degree=k, refine=2, perturbation=0.1, seed=5,
+            velocity_kind=1, factor_skew=0.5, irk_stages=s)
+    op = AdvectionOperation(c.dg())
+    nd = (k + 1) ** dim
+    N = op.n_dofs
+    r = np.random.default_rng(9).standard_normal(N * s)
+    dt = 0.05
+    op.set_time(0.0, dt)
+    z = op.initialize_dof_vector(blocks=s)
+    irk_fdm_apply(op, z, dev(r), prec)
+    cells = (0, 21, 42, op.n_cells - 1)
+    verts = op.cell_vertices()[list(cells)]
+    abar, det0 = cell_fdm_data(
+        verts, k, dim,
+        lambda x: np.array([np.sin(2 * np.pi * x[1]) * np.cos(2 * np.pi * x[2]),
+                            np.sin(2 * np.pi * x[2]) * np.cos(2 * np.pi * x[0]),
+                            np.sin(2 * np.pi * x[0]) * np.cos(2 * np.pi * x[1])]))
+    A, b, cc = irk_coefficients(s)
+    M, Ks = one_d_matrices(k, 1.0)
+    zz = host(z).reshape(s, -1, nd)
+    rr = r.reshape(s, -1, nd)
+    tol = 1e-10 if prec == 0 else fdm_fp32_tol(k, dim)
+    for j, cell in enumerate(cells):
+        Lsep = dense_cell_operator(M, Ks, abar[j], det0[j], 0.0, dim)
+        Mc = dense_cell_operator(M, Ks, [0, 0, 0], det0[j], 1.0, dim)
         Big = np.kron(np.linalg.inv(A) / dt, Mc) + np.kron(np.eye(s), Lsep)
-        zz = host(z).reshape(s, -1, nd)
-        rr = r.reshape(s, -1, nd)
-        for cell in (0, 17, op.n_cells - 1):
-            ref = np.linalg.solve(Big, rr[:, cell, :].ravel())
-            assert rel_diff(zz[:, cell, :].ravel(), ref) < tol
+        ref = np.linalg.solve(Big, rr[:, cell, :].ravel())
+        err = rel_diff(zz[:, cell, :].ravel(), ref)
+        print(f"perturbed fdm prec {prec} cell {cell} abar {abar[j]}: "
+              f"{err:.2e}")
+        assert err < tol
 
 
 # ---- solvers ----------------------------------------------------------------

@@ -101,3 +101,42 @@ def test_serve_request_config_without_gpu():
     bad = serve_request(4, 12 * 11, [1, 0, 0], [0.5] * 3, 0.1)
     with pytest.raises(_lib.DgError):
         request_config(bad)
+
+
+def test_failed_serving_request_is_released_with_its_error():
+    """Without a GPU every admission fails at its first device call: the
+    step reports the error, the request is FAILED and holds no memory, and
+    take_result releases it and returns the same error (CPU only; the
+    server itself makes no device call before admission)."""
+    lib = _lib.load()
+    srv = ctypes.c_void_p()
+    assert lib.dg_server_create(1 << 30, None, ctypes.byref(srv)) == _lib.DG_OK
+    r = _lib.DgServeRequest()
+    r.degree, r.cells_per_direction = 3, 8
+    r.velocity[0], r.width, r.final_time, r.courant = 1.0, 0.1, 0.1, 0.3
+    rid = ctypes.c_int64()
+    assert lib.dg_server_submit(srv, ctypes.byref(r), ctypes.byref(rid)) == 0
+    a, q, f = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    rc = lib.dg_server_step(srv, 1, ctypes.byref(a), ctypes.byref(q),
+                            ctypes.byref(f))
+    assert rc != _lib.DG_OK
+    step_error = lib.dg_last_error()
+    st, nd = ctypes.c_int(), ctypes.c_int64()
+    done, tot = ctypes.c_int(), ctypes.c_int()
+    assert lib.dg_server_request_info(srv, rid.value, ctypes.byref(st),
+                                      ctypes.byref(nd), ctypes.byref(done),
+                                      ctypes.byref(tot)) == 0
+    assert st.value == _lib.DG_REQUEST_FAILED
+    mem = ctypes.c_int64()
+    assert lib.dg_server_memory(srv, ctypes.byref(mem)) == 0
+    assert mem.value == 0
+    assert lib.dg_server_take_result(srv, rid.value, None) == \
+        _lib.DG_ERR_ARGUMENT                       # null destination
+    dst = ctypes.c_double()
+    assert lib.dg_server_take_result(srv, rid.value,
+                                     ctypes.byref(dst)) == rc
+    assert lib.dg_last_error().endswith(step_error)
+    lib.dg_server_request_info(srv, rid.value, ctypes.byref(st), None, None,
+                               None)
+    assert st.value == _lib.DG_REQUEST_UNKNOWN
+    assert lib.dg_server_destroy(srv) == 0
