@@ -141,6 +141,8 @@ SIGNATURES = {
                                     ctypes.POINTER(DgSolverControl), _dp]),
     "dg_vector_dot": (_i, [_vp, _i, _i64, _vp, _vp, _dp]),
     "dg_vector_multi_dot": (_i, [_vp, _i, _i64, _i, _vp, _vp, _dp]),
+    "dg_vector_multi_dot_noalloc": (_i, [_vp, _i, _i64, _i, _vp, _vp,
+                                         _dp]),
     "dg_vector_multi_axpy": (_i, [_vp, _i, _i64, _i, _dp, _vp, _vp]),
     "dg_vector_axpby": (_i, [_vp, _i, _i64, _d, _vp, _d, _vp]),
     "dg_vector_copy_convert": (_i, [_vp, _i64, _i, _vp, _i, _vp]),

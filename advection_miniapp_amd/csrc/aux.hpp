@@ -58,6 +58,11 @@ namespace dgops
   // Arnoldi steps with their scalars in device memory (d_ pointers), for
   // a Krylov method without host round trips; m <= 16 everywhere.
   // d_out[j] = <x_j, y>, j < m
+  // vec_multi_dot with its partials and sums in a static device array
+  // instead of allocated buffers (m <= 64): no allocation at all
+  void vec_multi_dot_noalloc(Operator &op, int p, int64_t n, int m,
+                             const void *x, int64_t xs, const void *y,
+                             double *result);
   void vec_multi_dot_dev(Operator &op, int p, int64_t n, int m,
                          const void *x, int64_t x_stride, const void *y,
                          double *d_out);

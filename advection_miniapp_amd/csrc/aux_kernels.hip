@@ -2,6 +2,10 @@
 // (mass, energy, H1 seminorm, L2 error), vector operations and the
 // pointwise updates of the time integrators. These run once per output or
 // are plain streaming kernels; the hot operator lives in op_kernel.hpp.
+#include <map>
+#include <mutex>
+#include <string>
+
 #include "aux.hpp"
 #include "op_kernel.hpp"
 #include "registry.hpp"
@@ -1355,6 +1359,115 @@ namespace dgops
     else
       multi_dot_dev_t<double>(op, n, m, (const double *)x, xs,
                               (const double *)y, d_out);
+  }
+
+  // ---- multi-dot without allocation ---------------------------------------
+  //
+  // dg_vector_multi_dot_noalloc keeps its block partials (one chunk of up to
+  // max_multi vectors over at most noalloc_blocks workgroups) and the m sums
+  // in this array, one copy per GPU for the life of the process, instead of
+  // the operator's allocated buffers. In host code `scratch` names no
+  // device address: kernels get the address hipGetSymbolAddress returns for
+  // the device (noalloc_scratch below). Every call on one GPU uses the same
+  // array, so calls that overlap on one GPU (two streams, two threads)
+  // race. Nothing in it has to start at zero: every partial and sum is
+  // written before it is read.
+  namespace
+  {
+    constexpr int noalloc_blocks  = 1024;
+    constexpr int noalloc_max_vec = 64;
+    constexpr int noalloc_doubles = noalloc_blocks * max_multi + noalloc_max_vec;
+
+    static __device__ __align__(256) double scratch[noalloc_doubles];
+
+    // the device address of `scratch` on `device`, looked up once per
+    // device with that device current
+    double *
+    noalloc_scratch(int device)
+    {
+      static std::mutex              mu;
+      static std::map<int, double *> ptrs;
+      std::lock_guard<std::mutex>    lock(mu);
+      auto                           it = ptrs.find(device);
+      if (it != ptrs.end())
+        return it->second;
+      int cur = 0;
+      hip_check(hipGetDevice(&cur), "hipGetDevice");
+      if (cur != device)
+        hip_check(hipSetDevice(device), "hipSetDevice");
+      void            *p  = nullptr;
+      const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (cur != device)
+        hip_check(hipSetDevice(cur), "hipSetDevice");
+      hip_check(rc, "hipGetSymbolAddress(scratch)");
+      if (p == nullptr)
+        throw Error(DG_ERR_HIP, "hipGetSymbolAddress(scratch): null address");
+      ptrs[device] = static_cast<double *>(p);
+      return ptrs[device];
+    }
+
+    template <typename T>
+    void
+    multi_dot_noalloc_t(Operator &op, double *scr, int64_t n, int m,
+                        const T *x, int64_t xs, const T *y, double *result)
+    {
+      const unsigned nb  = std::min<unsigned>(grid_for(n), noalloc_blocks);
+      double        *part = scr;
+      double        *sums = scr + noalloc_blocks * max_multi;
+      int            done = 0;
+      while (done < m)
+        {
+          const int chunk = std::min(max_multi, m - done);
+          const T  *xc    = x + done * xs;
+#define DG_MD(M)                                                              \
+  case M:                                                                     \
+    hipLaunchKernelGGL((multi_dot_kernel<T, M>), dim3(nb), dim3(vec_threads), \
+                       0, op.stream, n, xc, xs, y, part);                     \
+    break;
+          switch (chunk)
+            {
+              DG_MD(1) DG_MD(2) DG_MD(3) DG_MD(4) DG_MD(5) DG_MD(6) DG_MD(7)
+              DG_MD(8) DG_MD(9) DG_MD(10) DG_MD(11) DG_MD(12) DG_MD(13)
+              DG_MD(14) DG_MD(15) DG_MD(16)
+            }
+#undef DG_MD
+          hip_check(hipGetLastError(), "multi_dot");
+          hipLaunchKernelGGL(reduce_columns_small, dim3(1), dim3(1024), 0,
+                             op.stream, part, (int)nb, chunk, sums + done);
+          hip_check(hipGetLastError(), "reduce");
+          done += chunk;
+        }
+      hip_check(hipMemcpyAsync(result, sums, sizeof(double) * m,
+                               hipMemcpyDeviceToHost, op.stream),
+                "copy result");
+      hip_check(hipStreamSynchronize(op.stream), "sync");
+    }
+  } // namespace
+
+  void
+  vec_multi_dot_noalloc(Operator &op, int p, int64_t n, int m, const void *x,
+                        int64_t xs, const void *y, double *result)
+  {
+    if (m > noalloc_max_vec)
+      throw Error(DG_ERR_ARGUMENT,
+                  "multi_dot_noalloc: at most " +
+                    std::to_string(noalloc_max_vec) + " vectors, got " +
+                    std::to_string(m));
+    if (m <= 0)
+      return;
+    if (n <= 0)
+      {
+        for (int j = 0; j < m; ++j)
+          result[j] = 0;
+        return;
+      }
+    double *scr = noalloc_scratch(op.device);
+    if (p == DG_FP32)
+      multi_dot_noalloc_t<float>(op, scr, n, m, (const float *)x, xs,
+                                 (const float *)y, result);
+    else
+      multi_dot_noalloc_t<double>(op, scr, n, m, (const double *)x, xs,
+                                  (const double *)y, result);
   }
 
   void

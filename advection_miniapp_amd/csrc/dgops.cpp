@@ -777,6 +777,17 @@ dg_vector_multi_dot(dg_operator *h, int p, int64_t n, int m, const void *x,
 }
 
 int
+dg_vector_multi_dot_noalloc(dg_operator *h, int p, int64_t n, int m,
+                            const void *x, const void *y, double *result)
+{
+  return guard([&] {
+    require(x && y && result && m >= 0, "bad argument");
+    require(p == DG_FP64 || p == DG_FP32, "precision must be FP64 or FP32");
+    vec_multi_dot_noalloc(*get(h), p, n, m, x, n, y, result);
+  });
+}
+
+int
 dg_vector_multi_axpy(dg_operator *h, int p, int64_t n, int m,
                      const double *alpha, const void *x, void *y)
 {

@@ -315,6 +315,16 @@ int dg_vector_dot(dg_operator *op, int precision, int64_t n, const void *x,
 /* result[j] = <x_j, y> for j < m, x_j = x + j * n                       */
 int dg_vector_multi_dot(dg_operator *op, int precision, int64_t n, int m,
                         const void *x, const void *y, double *result);
+/* dg_vector_multi_dot without any device allocation: no counterpart in
+ * the reference. For callers that cannot allocate device memory (code
+ * under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc). The block partials and the sums live in a static device
+ * array of 131,584 bytes per GPU, held until the process ends; m <= 64
+ * (larger m is rejected with DG_ERR_ARGUMENT). Every call on one GPU
+ * shares that array: calls that overlap on one GPU race.                  */
+int dg_vector_multi_dot_noalloc(dg_operator *op, int precision, int64_t n,
+                                int m, const void *x, const void *y,
+                                double *result);
 /* y += sum_j alpha[j] x_j                                                */
 int dg_vector_multi_axpy(dg_operator *op, int precision, int64_t n, int m,
                          const double *alpha, const void *x, void *y);
