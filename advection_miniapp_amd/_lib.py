@@ -139,6 +139,8 @@ SIGNATURES = {
     "dg_irk_integrator_destroy": (_i, [_vp]),
     "dg_irk_integrator_step": (_i, [_vp, _d, _d, _vp,
                                     ctypes.POINTER(DgSolverControl), _dp]),
+    "dg_set_tile_shape": (_i, [_vp, _i]),
+    "dg_get_tile_shape": (_i, [_vp, ctypes.POINTER(_i), _dp, _dp]),
     "dg_vector_dot": (_i, [_vp, _i, _i64, _vp, _vp, _dp]),
     "dg_vector_multi_dot": (_i, [_vp, _i, _i64, _i, _vp, _vp, _dp]),
     "dg_vector_multi_dot_noalloc": (_i, [_vp, _i, _i64, _i, _vp, _vp,

@@ -315,8 +315,12 @@ namespace dgops
     Pk.bnd_index = m.d_bnd_index.as<int>();
     Pk.bnd_value = m.d_bnd_value.as<double>();
     aux_boundary_values(m, Pk);
-    const uint32_t key = op_key(op.cfg.dim, op.n, nc, fp32 ? 1 : 0,
-                                geo, skew ? 1 : 0, diff, fin, epi);
+    const uint32_t base = op_key(op.cfg.dim, op.n, nc, fp32 ? 1 : 0,
+                                 geo, skew ? 1 : 0, diff, fin, epi);
+    const TileChoice tile = tile_choice(m, base, Pk, fp32, epi, io);
+    m.last_tile           = tile;
+    m.last_tile_key       = base;
+    const uint32_t key    = base | (uint32_t)tile.shape << 19;
     auto it = op_registry().find(key);
     if (it == op_registry().end())
       throw Error(DG_ERR_UNSUPPORTED,
@@ -615,21 +619,13 @@ namespace dgops
     require(op.cfg.factor_skew == 0.,
             "explicit operator is built for factor_skew = 0",
             DG_ERR_UNSUPPORTED);
-    if (op.fp32())
-      {
-        OpBuffers<float> io = {};
-        io.src     = (const float *)ri_in;
-        io.sol_in  = (const float *)sol_in;
-        io.sol_out = (float *)sol_out;
-        io.ri_out  = (float *)ri_out;
-        io.b_dt    = (float)b_dt;
-        io.a_dt    = (float)a_dt;
-        io.last    = last;
-        launch_operator(op, P, 1, true, false, fin_inverse_mass, epi_lsrk,
-                        &io);
-      }
-    else
-      {
+    // the LSRK stage accumulates in FP64; FP32 operators give right-hand
+    // sides (apply) only
+    require(!op.fp32(),
+            "perform_stage: the fused LSRK stage is built in FP64 (create "
+            "the operator with precision DG_FP64)",
+            DG_ERR_UNSUPPORTED);
+    {
         OpBuffers<double> io = {};
         io.src     = (const double *)ri_in;
         io.sol_in  = (const double *)sol_in;

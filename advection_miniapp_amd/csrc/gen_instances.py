@@ -53,6 +53,29 @@ def op_variants():
     return v
 
 
+N_SHAPES = 3
+
+
+def tile_shapes(var):
+    """Tile shapes built for an operator variant: all three for the 3D
+    explicit right-hand sides (apply and fused LSRK stage, advection and
+    SIPG, FP64 and FP32), which the autotuner chooses among; the default
+    shape for the others."""
+    dim, n, nc, fp32, geo, skew, diff, fin, epi = var
+    if dim == 3 and nc == 1 and fin == FIN_INVMASS:
+        return range(N_SHAPES)
+    return (0,)
+
+
+def shape_cells(lines, shape):
+    """Cells per workgroup (shape_cells in op_kernel.hpp)."""
+    if shape == 1:
+        return 1 if lines >= 64 else 64 // lines
+    if shape == 2:
+        return 1 if lines >= 256 else 256 // lines
+    return 1 if lines >= 128 else 128 // lines
+
+
 def fdm_variants():
     v = []
     for dim, n, ns, prec in itertools.product((2, 3), range(2, 9), (1, 2, 3, 4),
@@ -90,8 +113,10 @@ def colloc_variants():
 
 
 def batch_variants():
-    # serving loop: batched explicit LSRK stages, 3D, degrees 1..7
-    return [("batch", 3, n) for n in range(2, 9)]
+    # serving loop: batched explicit LSRK stages, 3D, degrees 1..7, every
+    # tile shape
+    return [("batch", 3, n, sh) for n in range(2, 9)
+            for sh in range(N_SHAPES)]
 
 
 def b(x):
@@ -102,10 +127,11 @@ def emit(path, variants):
     lines = ['#include "../registry.hpp"', "", "namespace dgops", "{"]
     for i, var in enumerate(variants):
         if var[0] == "batch":
-            _, dim, n = var
+            _, dim, n, sh = var
+            cpb = shape_cells(n ** (dim - 1), sh)
             lines.append(
-                f"  static BatchRegistrar s{i}(batch_key({dim}, {n}), "
-                f"&launch_batch<{dim}, {n}>);")
+                f"  static BatchRegistrar s{i}(batch_key({dim}, {n}, {sh}), "
+                f"&launch_batch<{dim}, {n}, {cpb}>);")
             continue
         if var[0] == "colloc":
             _, dim, n, fp32 = var
@@ -141,12 +167,13 @@ def emit(path, variants):
                 f"  static FdmRegistrar f{i}(fdm_key({dim}, {n}, {ns}, {prec}), "
                 f"&launch_fdm<{dim}, {n}, {ns}, {tc}, {ti}, {to}>);")
             continue
-        dim, n, nc, fp32, geo, skew, diff, fin, epi = var
+        dim, n, nc, fp32, geo, skew, diff, fin, epi, sh = var
         T = "float" if fp32 else "double"
+        cpb = shape_cells(n ** (dim - 1), sh)
         lines.append(
             f"  static OpRegistrar r{i}(op_key({dim}, {n}, {nc}, {fp32}, {geo}, "
-            f"{skew}, {diff}, {fin}, {epi}), &launch_op<{dim}, {n}, {nc}, {T}, "
-            f"{geo}, {b(skew)}, {b(diff)}, {fin}, {epi}>);")
+            f"{skew}, {diff}, {fin}, {epi}, {sh}), &launch_op<{dim}, {n}, "
+            f"{nc}, {T}, {geo}, {b(skew)}, {b(diff)}, {fin}, {epi}, {cpb}>);")
     lines += ["} // namespace dgops", ""]
     text = "\n".join(lines)
     if os.path.exists(path) and open(path).read() == text:
@@ -166,7 +193,8 @@ def main(n_files=24):
         if v[0] in ("batch", "mass", "colloc", "cellgmres"):
             return v[2] ** (v[1] + 1)
         return v[1] ** (v[0] + 1) * v[2] * (2 if v[6] else 1)
-    variants = (op_variants() + fdm_variants() + burgers_variants() +
+    ops = [v + (sh,) for v in op_variants() for sh in tile_shapes(v)]
+    variants = (ops + fdm_variants() + burgers_variants() +
                 batch_variants() + mass_variants() + colloc_variants())
     # spread heavy (3D, large n) variants evenly
     variants.sort(key=lambda v: -weight(v))

@@ -87,6 +87,16 @@ namespace dgops
     std::vector<int>    pair;
   };
 
+  // tile shape of an operator launch (autotune.cpp): shape index (cells
+  // per workgroup, shape_cells in op_kernel.hpp) and the tuner's time of
+  // that candidate
+  struct TileChoice
+  {
+    int    shape = 0;
+    double ms    = 0;
+    bool   tuned = false;
+  };
+
   struct Operator
   {
     dg_config      cfg;
@@ -120,6 +130,12 @@ namespace dgops
     bool    boundary_ready = false;
     int64_t n_bnd          = 0; // boundary faces
 
+    // tile shapes: forced by dg_set_tile_shape (0 = autotuned), and the
+    // choice of the last operator launch with its variant key
+    int        tile_force_cells = 0;
+    TileChoice last_tile;
+    uint32_t   last_tile_key = 0;
+
     // scratch for reductions and solvers
     DevBuf d_partials, d_result;
     std::vector<DevBuf *> scratch_vectors;
@@ -144,6 +160,16 @@ namespace dgops
                        int             fin,
                        int             epi,
                        const void     *io);
+
+  // the tile shape for a launch of variant `base_key` (shape bits 0):
+  // forced, default, or tuned on first use per device and mesh size class
+  TileChoice tile_choice(Operator &op, uint32_t base_key, const OpParams &P,
+                         bool fp32, int epi, const void *io);
+  // the tuner's time of every shape (-1: not built) for the operator's
+  // mesh size class; false if the variant was not tuned
+  bool tile_candidates(const Operator &op, uint32_t base_key,
+                       double (&ms)[3]);
+  int  tile_cells(int dim, int n, int shape);
 
   double velocity_time_factor(const dg_config &cfg, double t);
 

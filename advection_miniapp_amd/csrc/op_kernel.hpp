@@ -146,6 +146,19 @@ namespace dgops
     return block_cells(Lay<dim, n>::nl);
   }
 
+  // Tile shapes of the operator kernels, chosen per kernel variant by the
+  // autotuner: shape 0 = the default above (two wavefronts), 1 = at most
+  // one wavefront of lines, 2 = at most four wavefronts.
+  constexpr int n_tile_shapes = 3;
+
+  constexpr int
+  shape_cells(int lines_per_cell, int shape)
+  {
+    return shape == 1 ? (lines_per_cell >= 64 ? 1 : 64 / lines_per_cell) :
+           shape == 2 ? (lines_per_cell >= 256 ? 1 : 256 / lines_per_cell) :
+                        block_cells(lines_per_cell);
+  }
+
   // ---- small register contractions ---------------------------------------
 
   // out[q] = sum_i M[q][i] in[i]
@@ -526,11 +539,11 @@ namespace dgops
   // L_c the advection operator at velocity vel_factor[c] * a(x).
   // With `diff` (axis-aligned cells) L also contains the SIPG diffusion
   // operator of advection_diffusion_solver.cc:531-613.
-  // The operator on the cells of workgroup `block` (one batch of
-  // cells_per_block cells); P and io live in kernel arguments or, for the
+  // The operator on the cells of workgroup `block` (one batch of CPB
+  // cells); P and io live in kernel arguments or, for the
   // batched launch, in global memory (uniform per workgroup either way).
   template <int dim, int n, int NC, typename T, int geo, bool skew,
-            bool diff, int fin, int epi>
+            bool diff, int fin, int epi, int CPB>
   __device__ __forceinline__ void
   operator_body(const OpParams                       &P,
                 const DevBasis<T> *__restrict__      Bp,
@@ -544,7 +557,6 @@ namespace dgops
     using L            = Lay<dim, n>;
     constexpr int nl   = L::nl;
     constexpr int nd   = L::nd;
-    constexpr int CPB  = cells_per_block<dim, n>();
     constexpr int STR  = L::stride;
     constexpr int FSTR = (diff ? 2 : 1) * 2 * dim * nl;
     constexpr int NV   = 1 << dim;
@@ -1252,15 +1264,20 @@ namespace dgops
       }
   }
 
+  // One batch of CPB cells per workgroup: workgroup b of n_blocks covers
+  // the cells of block xcd_block(b, n_blocks). (A grid-stride loop over
+  // several batches per workgroup was measured and rejected: the loop lets
+  // the compiler keep loop-invariant loads live across iterations, 107 ->
+  // 149 VGPRs at 3D degree 4, one occupancy step lost.)
   template <int dim, int n, int NC, typename T, int geo, bool skew,
-            bool diff, int fin, int epi>
-  __global__ void __launch_bounds__(128)
+            bool diff, int fin, int epi, int CPB>
+  __global__ void __launch_bounds__(CPB * (dim == 3 ? n * n : n))
   dg_operator_kernel(const OpParams                       P,
                      const DevBasis<T> *__restrict__      Bp,
                      const DevBasis<double> *__restrict__ Bd,
                      const OpBuffers<T>                   io)
   {
-    operator_body<dim, n, NC, T, geo, skew, diff, fin, epi>(
+    operator_body<dim, n, NC, T, geo, skew, diff, fin, epi, CPB>(
       P, Bp, Bd, io, xcd_block(blockIdx.x, gridDim.x));
   }
 
@@ -1279,8 +1296,8 @@ namespace dgops
   // advances all active solves of one degree by one LSRK stage with it).
   // Items are sorted by first_block; the lookup is uniform per workgroup.
   template <int dim, int n, int NC, typename T, int geo, bool skew,
-            bool diff, int fin, int epi>
-  __global__ void __launch_bounds__(128)
+            bool diff, int fin, int epi, int CPB>
+  __global__ void __launch_bounds__(CPB * (dim == 3 ? n * n : n))
   dg_operator_batch_kernel(const BatchItem<T> *__restrict__      items,
                            int                                  n_items,
                            const DevBasis<T> *__restrict__      Bp,
@@ -1292,7 +1309,7 @@ namespace dgops
       if (items[k].first_block <= b)
         r = k;
     const BatchItem<T> &it = items[r];
-    operator_body<dim, n, NC, T, geo, skew, diff, fin, epi>(
+    operator_body<dim, n, NC, T, geo, skew, diff, fin, epi, CPB>(
       it.P, Bp, Bd, it.io, b - it.first_block);
   }
 } // namespace dgops

@@ -16,6 +16,7 @@
 
 namespace dgops
 {
+  // launcher of one operator variant and tile shape
   using OpLaunch = hipError_t (*)(const OpParams &,
                                   const void *basis_t,
                                   const void *basis_d,
@@ -24,12 +25,12 @@ namespace dgops
 
   inline uint32_t
   op_key(int dim, int n, int nc, int fp32, int geo, int skew, int diff,
-         int fin, int epi)
+         int fin, int epi, int shape = 0)
   {
     return (uint32_t)dim | (uint32_t)n << 2 | (uint32_t)nc << 6 |
            (uint32_t)fp32 << 10 | (uint32_t)geo << 11 |
            (uint32_t)skew << 13 | (uint32_t)fin << 14 | (uint32_t)epi << 16 |
-           (uint32_t)diff << 18;
+           (uint32_t)diff << 18 | (uint32_t)shape << 19;
   }
 
   std::map<uint32_t, OpLaunch> &op_registry();
@@ -256,9 +257,10 @@ namespace dgops
                                      hipStream_t stream);
 
   inline uint32_t
-  batch_key(int dim, int n)
+  batch_key(int dim, int n, int shape = 0)
   {
-    return (uint32_t)dim | (uint32_t)n << 2 | 1u << 21;
+    return (uint32_t)dim | (uint32_t)n << 2 | (uint32_t)shape << 6 |
+           1u << 21;
   }
 
   std::map<uint32_t, BatchLaunch> &batch_registry();
@@ -268,19 +270,19 @@ namespace dgops
     BatchRegistrar(uint32_t key, BatchLaunch f) { batch_registry()[key] = f; }
   };
 
-  template <int dim, int n>
+  template <int dim, int n, int CPB>
   hipError_t
   launch_batch(const void *items, int n_items, unsigned blocks,
                const void *basis, hipStream_t stream)
   {
-    constexpr int CPB = cells_per_block<dim, n>();
-    constexpr int nl  = Lay<dim, n>::nl;
-    static_assert(CPB * nl <= 128, "kernels are bounded to 128 threads");
+    constexpr int nl = Lay<dim, n>::nl;
+    static_assert(CPB * nl <= 256, "kernels are bounded to 256 threads");
     if (blocks == 0 || n_items <= 0)
       return hipSuccess;
     hipLaunchKernelGGL((dg_operator_batch_kernel<dim, n, 1, double,
                                                  geo_uniform, false, false,
-                                                 fin_inverse_mass, epi_lsrk>),
+                                                 fin_inverse_mass, epi_lsrk,
+                                                 CPB>),
                        dim3(blocks), dim3(CPB * nl), 0, stream,
                        (const BatchItem<double> *)items, n_items,
                        (const DevBasis<double> *)basis,
@@ -289,7 +291,7 @@ namespace dgops
   }
 
   template <int dim, int n, int NC, typename T, int geo, bool skew,
-            bool diff, int fin, int epi>
+            bool diff, int fin, int epi, int CPB>
   hipError_t
   launch_op(const OpParams &P,
             const void     *bt,
@@ -297,14 +299,13 @@ namespace dgops
             const void     *io,
             hipStream_t     stream)
   {
-    constexpr int CPB    = cells_per_block<dim, n>();
     constexpr int nl     = Lay<dim, n>::nl;
-    static_assert(CPB * nl <= 128, "kernels are bounded to 128 threads");
+    static_assert(CPB * nl <= 256, "kernels are bounded to 256 threads");
     const int64_t blocks = (P.n_cells + CPB - 1) / CPB;
     if (blocks <= 0)
       return hipSuccess;
     hipLaunchKernelGGL(
-      (dg_operator_kernel<dim, n, NC, T, geo, skew, diff, fin, epi>),
+      (dg_operator_kernel<dim, n, NC, T, geo, skew, diff, fin, epi, CPB>),
       dim3((unsigned)blocks),
       dim3(CPB * nl),
       0,

@@ -117,6 +117,7 @@ namespace dgops
       int64_t          op_bytes, vec_bytes;
       int              err_code = DG_OK; // setup error of a FAILED request
       std::string      err_msg;
+      int              shape = 0; // tile shape of its LSRK stage kernel
 
       Operator *op() const { return reinterpret_cast<Operator *>(handle); }
       ~ServeReq()
@@ -175,6 +176,23 @@ namespace dgops
       q.err_msg  = "request " + std::to_string(q.id) + " setup: " + msg;
     }
 
+    // the tile shape of the request's fused LSRK stage kernel, tuned (or
+    // found in the tuner's table) with its projected solution as input and
+    // scratch outputs, as a solve run alone would tune it
+    static int
+    tuned_shape(ServeReq &q)
+    {
+      const OpParams    P  = q.op()->base_params(0., 1);
+      OpBuffers<double> io = {};
+      io.src     = q.reg[0].as<double>();
+      io.sol_in  = q.reg[0].as<double>();
+      io.sol_out = q.reg[1].as<double>();
+      io.ri_out  = q.reg[2].as<double>();
+      const uint32_t base = op_key(3, q.cfg.degree + 1, 1, 0, geo_uniform,
+                                   0, 0, fin_inverse_mass, epi_lsrk);
+      return tile_choice(*q.op(), base, P, false, epi_lsrk, &io).shape;
+    }
+
     void
     admit()
     {
@@ -195,6 +213,7 @@ namespace dgops
               for (auto &r : q.reg)
                 r.alloc(q.vec_bytes);
               aux_project(*q.op(), q.reg[0].ptr, 0.);
+              q.shape = tuned_shape(q);
             }
           catch (const Error &e)
             {
@@ -259,13 +278,18 @@ namespace dgops
         for (const LsrkStage &s : plan)
           for (auto &g : groups)
             {
-              const int n  = g.first;
-              auto      it = batch_registry().find(batch_key(3, n));
+              const int n = g.first;
+              // the tile shape tuned for the group's largest solve
+              const ServeReq *big = g.second.front();
+              for (const ServeReq *q : g.second)
+                if (q->n_cells > big->n_cells)
+                  big = q;
+              auto it = batch_registry().find(batch_key(3, n, big->shape));
               require(it != batch_registry().end(),
                       "batched operator not built for degree " +
                         std::to_string(n - 1),
                       DG_ERR_UNSUPPORTED);
-              const int64_t cpb = block_cells(n * n);
+              const int64_t cpb = tile_cells(3, n, big->shape);
               Launch        L   = {it->second, k, 0, 0,
                                    g.second.front()->op()->d_basis_d.ptr};
               unsigned      first = 0;

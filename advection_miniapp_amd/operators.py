@@ -130,6 +130,23 @@ class AdvectionOperation:
             self._h, scheme, t, dt, self._v(solution, "solution"),
             self._v(vec_ri, "vec_ri"), self._v(vec_ki, "vec_ki")))
 
+    def set_tile_shape(self, cells_per_workgroup=0):
+        """Force the operator kernels' tile shape; 0 = autotuned."""
+        L.check(self._lib.dg_set_tile_shape(self._h, cells_per_workgroup))
+
+    def tile_shape(self):
+        """The tile shape of the last operator launch: cells per
+        workgroup, the tuned time (ms, 0 if not tuned) and the tuner's
+        time of every shape {cells per workgroup: ms, -1 if not timed}."""
+        c = ctypes.c_int()
+        ms = ctypes.c_double()
+        cand = (ctypes.c_double * 6)()
+        L.check(self._lib.dg_get_tile_shape(self._h, ctypes.byref(c),
+                                            ctypes.byref(ms), cand))
+        return dict(cells_per_workgroup=c.value, tuned_ms=ms.value,
+                    candidates_ms={int(cand[2 * i]): cand[2 * i + 1]
+                                   for i in range(3)})
+
     def project_initial(self, dst, time=0.0):
         L.check(self._lib.dg_project_initial(self._h, self._v(dst, "dst"),
                                              time))

@@ -20,7 +20,8 @@ around the FDM), each in FP64 and with the preconditioner in FP32; the
 speedup is the fastest all-FP64 solve over the fastest mixed one, with the
 same-algorithm ratios beside it.
 
-Also by default: configuration 1 (2D run, L2 error), configuration 3
+Also by default: the apply workload at degrees 1..7 with the autotuned
+tile shapes, configuration 1 (2D run, L2 error), configuration 3
 (SIPG, degrees 3/5/7, FP64 and FP32), the Burgers add-on of configuration 4,
 the serving stream (configuration 5) against serial solves, and the CPU
 baseline (the oracle on the host cores).
@@ -131,6 +132,7 @@ def time_apply(args):
     stop.record(stream)
     torch.cuda.synchronize()
     ms = start.elapsed_time(stop) / args.steps
+    op.tile = op.tile_shape()
     return op, ms
 
 
@@ -340,7 +342,8 @@ def time_apply_by_degree():
         op, src, dst, ms = time_one_apply(apply_workload(refine=r, degree=k))
         n = op.n_dofs
         out[f"k{k}"] = dict(n_dofs=n, ms=ms, dof_per_s=n / (ms * 1e-3),
-                            roofline_frac=16 * n / (ms * 1e-3) / 8e12)
+                            roofline_frac=16 * n / (ms * 1e-3) / 8e12,
+                            tile=op.tile_shape())
         del op, src, dst
         torch.cuda.empty_cache()
     return out
@@ -558,15 +561,19 @@ def time_config3():
                     solution_param=[2 * math.pi, 1e-3])
         op, src, dst64, ms64 = time_one_apply(base)
         n = op.n_dofs
+        tile64 = op.tile_shape()
         del op
-        _, _, dst32, ms32 = time_one_apply(dict(base, precision=1),
-                                           src=src.float())
+        op32, _, dst32, ms32 = time_one_apply(dict(base, precision=1),
+                                              src=src.float())
+        tile32 = op32.tile_shape()
+        del op32
         err = ((dst32.double() - dst64).abs().max() /
                dst64.abs().max()).item()
         out[f"k{k}"] = dict(
             n_dofs=n, fp64_ms=ms64, fp64_dof_per_s=n / (ms64 * 1e-3),
             fp32_ms=ms32, fp32_dof_per_s=n / (ms32 * 1e-3),
-            fp32_rel_err_vs_fp64=err)
+            fp32_rel_err_vs_fp64=err,
+            tile_fp64=tile64, tile_fp32=tile32)
         del src, dst64, dst32
         torch.cuda.empty_cache()
     return out
@@ -589,11 +596,10 @@ def time_burgers():
 
 
 def time_extra():
-    """Further measurements: the configuration-2 apply by degree, the
-    configuration-2 run, the standalone inverse mass, the batched cell
-    solver and the implicit operator parts."""
+    """Further measurements: the configuration-2 run, the standalone
+    inverse mass, the batched cell solver and the implicit operator
+    parts."""
     out = {}
-    out["apply_by_degree"] = time_apply_by_degree()
     out["config2_run"] = time_config2_run()
     out["inverse_mass"] = time_inverse_mass()
     out["cell_solver"] = time_cell_solver()
@@ -670,11 +676,13 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "fp64_tflops": APPLY_FLOP_PER_DOF * n / (ms * 1e-3) / 1e12,
+        "tile": op.tile,
     }
     del op
     if not args.no_irk:
         out["irk"] = time_irk(args)
     if not args.no_configs:
+        out["apply_by_degree"] = time_apply_by_degree()
         out["config1"] = time_config1_run()
         out["config3_sipg"] = time_config3()
         out["config4_burgers"] = time_burgers()

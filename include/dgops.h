@@ -146,6 +146,24 @@ int dg_destroy(dg_operator *op);
 int dg_get_info(const dg_operator *op, dg_info *info);
 int dg_set_stream(dg_operator *op, void *stream);
 
+/* ---- tile shapes of the operator kernels (per-degree autotuner) -------
+ * The explicit right-hand side kernels of 3D operators (apply and
+ * perform_stage, advection and SIPG, FP64 and FP32) are built for three
+ * tile shapes: one, two or four wavefronts of cell lines per workgroup.
+ * By default the first launch of such a variant on a device (meshes of
+ * 4096 cells and more) times every shape with HIP events on the handle's
+ * stream, into scratch vectors, and keeps the fastest for the process.
+ * The shape changes the time, never the result.
+ *                  replaces MatrixFree's batch setup advection_solver.cc:436-465 */
+/* force the cells per workgroup (one of the shapes of the degree); 0
+ * restores the autotuner                                                */
+int dg_set_tile_shape(dg_operator *op, int cells_per_workgroup);
+/* the shape of the handle's last operator launch and its tuned time (0
+ * when not tuned); candidates (may be null) receives 3 pairs {cells per
+ * workgroup, tuned ms}, ms -1 where a shape was not built or not timed  */
+int dg_get_tile_shape(const dg_operator *op, int *cells_per_workgroup,
+                      double *tuned_ms, double *candidates);
+
 /* ---- explicit operator ------------------------------------------------ */
 
 /* dst = M^-1 (-L(t) src)            AdvectionOperation::apply
