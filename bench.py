@@ -110,10 +110,20 @@ def measured_traffic():
         return json.load(f).get("bytes_per_launch")
 
 
+def uniform_apply_workload(refine=6, degree=4):
+    """The apply workload's mesh size and degree on axis-aligned cells with
+    a constant speed (1, 0.5, 0.25): the operator kernel without geometry
+    tables, for profiling against the tabulated one."""
+    return dict(apply_workload(refine=refine, degree=degree),
+                perturbation=0.0, velocity_kind=0, velocity=[1.0, 0.5, 0.25])
+
+
 def time_apply(args):
     import torch
     from advection_miniapp_amd import AdvectionOperation, make_config
-    op = AdvectionOperation(make_config(**apply_workload(refine=args.refine)))
+    wl = (uniform_apply_workload if args.apply_geometry == "uniform"
+          else apply_workload)
+    op = AdvectionOperation(make_config(**wl(refine=args.refine)))
     src = op.initialize_dof_vector()
     op.project_initial(src, 0.0)
     dst = op.initialize_dof_vector()
@@ -613,6 +623,11 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--refine", type=int, default=6)
+    ap.add_argument("--apply-geometry", choices=["config2", "uniform"],
+                    default="config2",
+                    help="uniform: time the apply on the same mesh size "
+                         "with axis-aligned cells and a constant speed "
+                         "(no geometry tables), for profiling")
     ap.add_argument("--irk-refine", type=int, default=5)
     ap.add_argument("--irk-reps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
@@ -670,7 +685,10 @@ def main():
         "data": "synthetic",
         "config": {"workload": "3D advection apply, degree 4, 64^3 cells "
                                "perturbed 10% (seed 42), cellular velocity, "
-                               "FP64",
+                               "FP64" if args.apply_geometry == "config2" else
+                               "3D advection apply, degree 4, 64^3 "
+                               "axis-aligned cells, constant velocity (no "
+                               "geometry tables), FP64",
                    "n_dofs": n, "refine": args.refine},
         "roofline": {"bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
