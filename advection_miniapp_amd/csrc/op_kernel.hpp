@@ -1005,6 +1005,11 @@ namespace dgops
     // speed rows of the y and x phases, read once for all components
     if constexpr (geo == geo_table)
       load_speed_rows();
+    // tabulated inverse mass: the 1/JxW row is requested after the LAST
+    // phase (one component; several components read it per component
+    // after their x phase)
+    constexpr bool early_ij = geo == geo_table && fin == fin_inverse_mass;
+    T ij_row[early_ij ? n : 1];
     if constexpr (UNI)
       {
 #pragma unroll
@@ -1055,6 +1060,16 @@ namespace dgops
           face_terms<dim, n, LAST, T, diff, fin == fin_local>(P, Bd, B, cell, p, active, nb,
                                             uplus[c], dplus[c], uqs[c], acc,
                                             vf, fan);
+          if constexpr (early_ij)
+            {
+              // the 1/JxW row of the inverse mass, requested here (the
+              // LAST speed row is dead) so that it arrives during the y
+              // and x phases instead of after them; no extra registers at
+              // the kernel's peak (3D degree 4: 107 VGPRs either way)
+              __builtin_amdgcn_s_setprio(3);
+              table_line<dim, n>(tab_c, cell, dim, p, active, ij_row);
+              __builtin_amdgcn_s_setprio(0);
+            }
           lds_write_line<dim, n, LAST>(u_l, p, uqs[c]);
           lds_write_line<dim, n, LAST>(acc_l, p, acc);
           if constexpr (AN)
@@ -1198,7 +1213,9 @@ namespace dgops
                   ij[i] = B.winv[i] * r;
               }
             else
-              table_line<dim, n>(tab_c, cell, dim, p, active, ij);
+#pragma unroll
+              for (int i = 0; i < n; ++i)
+                ij[i] = ij_row[i];
 #pragma unroll
             for (int i = 0; i < n; ++i)
               acc[i] *= ij[i];
