@@ -1,0 +1,574 @@
+// Inviscid Burgers right-hand side M^-1(-L(u)) of a 3-component field in 3D
+// for degrees 1..7 (burgers_solver.cc:259-341, the degree a run-time
+// argument there, :936-938): cell flux u (x) u and the Rusanov face flux
+//   F = 1/2 [(u-.n) u- + (u+.n) u+ + alpha max(|u-.n|, |u+.n|)(u- - u+)]
+// integrated with QGauss(m), m = k(k+1)/2 (burgers_solver.cc:234), and the
+// inverse mass matrix of QGauss(k+1) (:235, :366).
+//
+// The m^3 quadrature tensors (28^3 points at degree 7) do not fit in LDS,
+// so the cell term runs slab by slab: the nodal values are first contracted
+// along z to the m quadrature levels (W), and for every level qz the slab
+// is interpolated in x and y, the fluxes u_c v_d (v = w adj(J) u) are formed
+// point-wise, integrated in x and y, and the z integration is accumulated
+// in registers of the threads that own the cell's z-lines:
+//   res_c = sum_qz S_z[qz] (x) (S_y^T G_x^T f_cx + G_y^T S_x^T f_cy)
+//         + G_z[qz] (x) S_y^T S_x^T f_cz.
+// Faces are element-centric (own outward normal, neighbour trace from the
+// neighbour's nodal face layer) and are integrated first into an LDS
+// accumulator that seeds the registers. CPB cells per workgroup, R threads
+// per cell; every phase loops its tasks over the cell's R threads.
+#pragma once
+
+#include "burgers_kernel.hpp"
+
+namespace dgops
+{
+  constexpr int
+  burgers3_threads(int n, int m)
+  {
+    const int a = m * m, b = n * m, c = n * n;
+    return a > b ? (a > c ? a : c) : (b > c ? b : c);
+  }
+
+  // cells per workgroup: up to 128 threads of cells
+  constexpr int
+  burgers3_cells(int n, int m)
+  {
+    const int r = burgers3_threads(n, m);
+    return r >= 128 ? 1 : 128 / r;
+  }
+
+  template <int f>
+  struct FaceId
+  {
+    static constexpr int value = f;
+  };
+
+  // face f = 2 d + s of a cell: its normal direction d, side s, tangential
+  // directions ta < tb, and node (a, b) of the layer at `layer` along d in
+  // the lexicographic cell numbering (x fastest)
+  template <int n, int f>
+  struct FaceIdx
+  {
+    static constexpr int d = f / 2, s = f % 2;
+    static constexpr int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;
+    static constexpr int sd = d == 0 ? 1 : d == 1 ? n : n * n;
+    static constexpr int sa = ta == 0 ? 1 : ta == 1 ? n : n * n;
+    static constexpr int sb = tb == 0 ? 1 : tb == 1 ? n : n * n;
+    __device__ static constexpr int
+    node(int a, int b, int layer)
+    {
+      return layer * sd + a * sa + b * sb;
+    }
+  };
+
+  template <int n, int m>
+  struct Burgers3Lay
+  {
+    static constexpr int n2 = n * n, n3 = n * n * n, m2 = m * m;
+    static constexpr int R  = burgers3_threads(n, m);
+    // per-cell LDS offsets (doubles)
+    static constexpr int oU  = 0;                // [3][n3] nodal input
+    static constexpr int oRF = oU + 3 * n3;      // [3][n3] face terms
+    static constexpr int oW  = oRF + 3 * n3;     // [3][m][n2]
+    static constexpr int oT  = oW + 3 * m * n2;  // [3][n][m]
+    static constexpr int oUV = oT + 3 * n * m;   // [6][m2]
+    static constexpr int oF  = oUV + 6 * m2;     // [9][m][n]
+    static constexpr int oX  = oF + 9 * m * n;   // [8][3] corners
+    static constexpr int oNB = oX + 24;          // [6][3][n2] neighbour
+                                                 // face layers
+    static constexpr int size = oNB + 18 * n2;
+  };
+
+  template <int n, int m, bool cartesian, int epi, int CPB>
+  __global__ void __launch_bounds__(CPB * burgers3_threads(n, m))
+  burgers3_kernel(const OpParams P, const DevBasis<double> *__restrict__ Bd,
+                  const BurgersTables *__restrict__ BT,
+                  const OpBuffers<double> io)
+  {
+    using L             = Burgers3Lay<n, m>;
+    constexpr int n2    = L::n2, n3 = L::n3, m2 = L::m2, R = L::R;
+    constexpr int nvec  = 3 * n3;
+    __shared__ double lds[CPB * L::size];
+    __shared__ double sS[m * n], sG[m * n], sw[m], sx[m];
+
+    const int     tid    = threadIdx.x;
+    const int     lc     = tid / R;
+    const int     r      = tid - lc * R;
+    const int64_t cell   = (int64_t)xcd_block(blockIdx.x, gridDim.x) * CPB +
+                         lc;
+    const bool    active = cell < P.n_cells;
+    double       *U  = lds + lc * L::size + L::oU;
+    double       *RF = lds + lc * L::size + L::oRF;
+    double       *W  = lds + lc * L::size + L::oW;
+    double       *T  = lds + lc * L::size + L::oT;
+    double       *UV = lds + lc * L::size + L::oUV;
+    double       *F  = lds + lc * L::size + L::oF;
+    double       *X  = lds + lc * L::size + L::oX;
+    double       *NB = lds + lc * L::size + L::oNB;
+    const double *src = io.src + cell * nvec;
+
+    // ---- inputs: nodal values, corners, the QGauss(m) tables -----------
+    for (int e = tid; e < m * n; e += blockDim.x)
+      {
+        sS[e] = BT->Sm[e / n][e % n];
+        sG[e] = BT->Gm[e / n][e % n];
+      }
+    for (int e = tid; e < m; e += blockDim.x)
+      {
+        sw[e] = BT->wm[e];
+        sx[e] = BT->xm[e];
+      }
+    for (int e = r; e < nvec; e += R)
+      {
+        U[e]  = active ? src[e] : 0.;
+        RF[e] = 0.;
+      }
+    if (!cartesian)
+      for (int e = r; e < 24; e += R)
+        X[e] = active ? P.cell_vertices[cell * 24 + e] : 0.;
+    // the neighbours' layers facing this cell, all six faces before the
+    // first barrier: one round trip instead of one per face.
+    // NB[(f 3 + c) n2 + b n + a] is node (a, b) of face f (a along the first,
+    // b along the second tangential direction) of component c
+    auto load_nb = [&](auto fc) {
+      constexpr int f  = decltype(fc)::value;
+      using FI         = FaceIdx<n, f>;
+      const int64_t nb =
+        active ? (int64_t)P.neighbors[cell * 6 + f] : (int64_t)-1;
+      for (int e = r; e < 3 * n2; e += R)
+        {
+          const int c = e / n2, a = e % n, b = (e / n) % n;
+          NB[(f * 3 + c) * n2 + b * n + a] =
+            nb >= 0 ?
+              io.src[nb * nvec + c * n3 + FI::node(a, b, FI::s ? 0 : n - 1)] :
+              0.;
+        }
+    };
+    load_nb(FaceId<0>{});
+    load_nb(FaceId<1>{});
+    load_nb(FaceId<2>{});
+    load_nb(FaceId<3>{});
+    load_nb(FaceId<4>{});
+    load_nb(FaceId<5>{});
+    __syncthreads();
+
+    // ---- faces: own and neighbour traces at the m x m face points, the
+    // Rusanov flux with this cell's outward normal, integrated onto the own
+    // face layer (RF). Both faces of a direction at once (their layers are
+    // disjoint); one instance per direction, so the strides are static. The
+    // face buffers overlay the cell-term buffers W..F, unused until the
+    // cell term: FT [side s][own/neighbour][c][b m + qa] (12 n m values),
+    // FUV [s][own/neighbour][c][qb m + qa] (12 m^2 values)
+    static_assert(12 * n * m + 12 * m2 <= L::oX - L::oW,
+                  "face buffers overlay the cell-term buffers");
+    double *FT  = W;
+    double *FUV = W + 12 * n * m;
+    auto    face_pair = [&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      using FI        = FaceIdx<n, 2 * d>; // strides of direction d
+      constexpr int ta = FI::ta, tb = FI::tb;
+      // along ta
+      for (int t = r; t < 12 * n * m; t += R)
+        {
+          const int k = t / (n * m); // (s 2 + side) 3 + c
+          const int b = (t / m) % n, qa = t % m;
+          const int s = k / 6, sd = (k / 3) % 2, c = k % 3;
+          double    v = 0;
+          if (sd == 0)
+            {
+              const int layer = s ? n - 1 : 0;
+#pragma unroll
+              for (int a = 0; a < n; ++a)
+                v = fma(sS[qa * n + a], U[c * n3 + FI::node(a, b, layer)], v);
+            }
+          else
+#pragma unroll
+            for (int a = 0; a < n; ++a)
+              v = fma(sS[qa * n + a],
+                      NB[((2 * d + s) * 3 + c) * n2 + b * n + a], v);
+          FT[k * n * m + b * m + qa] = v;
+        }
+      __syncthreads();
+      // along tb
+      for (int t = r; t < 12 * m2; t += R)
+        {
+          const int k  = t / m2;
+          const int qb = (t / m) % m, qa = t % m;
+          double    v  = 0;
+#pragma unroll
+          for (int b = 0; b < n; ++b)
+            v = fma(sS[qb * n + b], FT[k * n * m + b * m + qa], v);
+          FUV[k * m2 + qb * m + qa] = v;
+        }
+      __syncthreads();
+      // flux at the face points; the own slots receive -F n dA
+      for (int t = r; t < 2 * m2; t += R)
+        {
+          const int    s  = t / m2, pt = t % m2;
+          const int    qa = pt % m, qb = pt / m;
+          const double w  = sw[qa] * sw[qb];
+          double       nrm[3] = {0, 0, 0};
+          if constexpr (cartesian)
+            {
+              double area = 1;
+              for (int e = 0; e < 3; ++e)
+                if (e != d)
+                  area *= P.h[e];
+              nrm[d] = (s ? 1. : -1.) * area * w;
+            }
+          else
+            {
+              double xi[3];
+              xi[d]  = s;
+              xi[ta] = sx[qa];
+              xi[tb] = sx[qb];
+              double x[3], J[3][3], adj[3][3];
+              multilinear_map<3>(X, xi, x, J);
+              adjugate<3>(J, adj);
+              for (int e = 0; e < 3; ++e)
+                nrm[e] = (s ? 1. : -1.) * adj[d][e] * w;
+            }
+          const double len =
+            sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+          double *own = FUV + s * 6 * m2, *nbr = own + 3 * m2;
+          double  um[3], up[3], unm = 0, unp = 0;
+          for (int c = 0; c < 3; ++c)
+            {
+              um[c] = own[c * m2 + pt];
+              up[c] = nbr[c * m2 + pt];
+              unm += um[c] * nrm[c] / len;
+              unp += up[c] * nrm[c] / len;
+            }
+          const double lam = P.alpha * fmax(fabs(unm), fabs(unp));
+          for (int c = 0; c < 3; ++c)
+            own[c * m2 + pt] =
+              -0.5 * (unm * um[c] + unp * up[c] + lam * (um[c] - up[c])) * len;
+        }
+      __syncthreads();
+      // integrate: along tb (into FT[s 3 + c]), then along ta onto the own
+      // layer of side s
+      for (int t = r; t < 6 * n * m; t += R)
+        {
+          const int k = t / (n * m); // s 3 + c
+          const int b = (t / m) % n, qa = t % m;
+          const double *fl = FUV + ((k / 3) * 6 + k % 3) * m2;
+          double        v  = 0;
+#pragma unroll 4
+          for (int qb = 0; qb < m; ++qb)
+            v = fma(sS[qb * n + b], fl[qb * m + qa], v);
+          FT[k * n * m + b * m + qa] = v;
+        }
+      __syncthreads();
+      for (int t = r; t < 6 * n2; t += R)
+        {
+          const int k = t / n2; // s 3 + c
+          const int b = (t / n) % n, a = t % n;
+          double    v = 0;
+#pragma unroll 4
+          for (int qa = 0; qa < m; ++qa)
+            v = fma(sS[qa * n + a], FT[k * n * m + b * m + qa], v);
+          RF[(k % 3) * n3 + FI::node(a, b, k / 3 ? n - 1 : 0)] += v;
+        }
+      __syncthreads();
+    };
+    face_pair(FaceId<0>{});
+    face_pair(FaceId<1>{});
+    face_pair(FaceId<2>{});
+
+    // ---- cell term, slab by slab ----------------------------------------
+    // W[c][qz][j n + i] = sum_k S[qz][k] u[c][k][j][i]
+    for (int t = r; t < 3 * n2; t += R)
+      {
+        const int c = t / n2, ij = t % n2;
+        double    u[n];
+#pragma unroll
+        for (int k = 0; k < n; ++k)
+          u[k] = U[c * n3 + k * n2 + ij];
+#pragma unroll 2
+        for (int qz = 0; qz < m; ++qz)
+          {
+            double v = 0;
+#pragma unroll
+            for (int k = 0; k < n; ++k)
+              v = fma(sS[qz * n + k], u[k], v);
+            W[(c * m + qz) * n2 + ij] = v;
+          }
+      }
+    // z-line owners (thread r < n2: i = r % n, j = r / n) accumulate
+    double acc[3][n];
+    {
+      const int ij = r < n2 ? r : 0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int k = 0; k < n; ++k)
+          acc[c][k] = RF[c * n3 + k * n2 + ij];
+    }
+    __syncthreads();
+    double vol = 0, adiag[3] = {0, 0, 0};
+    if constexpr (cartesian)
+      {
+        vol = P.h[0] * P.h[1] * P.h[2];
+        for (int e = 0; e < 3; ++e)
+          adiag[e] = vol / P.h[e];
+      }
+#pragma unroll 1
+    for (int qz = 0; qz < m; ++qz)
+      {
+        // along x: T[c][j m + qx]
+        for (int t = r; t < 3 * n * m; t += R)
+          {
+            const int c = t / (n * m), j = (t / m) % n, qx = t % m;
+            double    v = 0;
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+              v = fma(sS[qx * n + i], W[(c * m + qz) * n2 + j * n + i], v);
+            T[c * n * m + j * m + qx] = v;
+          }
+        __syncthreads();
+        // along y, then the point-wise fluxes: UV = (u_0, u_1, u_2, v_0,
+        // v_1, v_2) with v = w adj(J) u
+        for (int t = r; t < m2; t += R)
+          {
+            const int qx = t % m, qy = t / m;
+            double    u[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+              {
+                double v = 0;
+#pragma unroll
+                for (int j = 0; j < n; ++j)
+                  v = fma(sS[qy * n + j], T[c * n * m + j * m + qx], v);
+                u[c] = v;
+              }
+            const double w = sw[qx] * sw[qy] * sw[qz];
+            double       v[3];
+            if constexpr (cartesian)
+              {
+                for (int e = 0; e < 3; ++e)
+                  v[e] = w * adiag[e] * u[e];
+              }
+            else
+              {
+                const double xi[3] = {sx[qx], sx[qy], sx[qz]};
+                double       x[3], J[3][3], adj[3][3];
+                multilinear_map<3>(X, xi, x, J);
+                adjugate<3>(J, adj);
+                for (int e = 0; e < 3; ++e)
+                  v[e] = w * (adj[e][0] * u[0] + adj[e][1] * u[1] +
+                              adj[e][2] * u[2]);
+              }
+            for (int c = 0; c < 3; ++c)
+              {
+                UV[c * m2 + t]       = u[c];
+                UV[(3 + c) * m2 + t] = v[c];
+              }
+          }
+        __syncthreads();
+        // integrate along x: F[c 3 + branch][qy n + i], branches d = x (G),
+        // y (S), z (S)
+        for (int t = r; t < n * m; t += R)
+          {
+            const int i = t % n, qy = t / n;
+            double    a[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll 2
+            for (int qx = 0; qx < m; ++qx)
+              {
+                const int    pt = qy * m + qx;
+                const double g  = sG[qx * n + i], sv = sS[qx * n + i];
+                const double c0 = g * UV[3 * m2 + pt];
+                const double c1 = sv * UV[4 * m2 + pt];
+                const double c2 = sv * UV[5 * m2 + pt];
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                  {
+                    const double uc = UV[c * m2 + pt];
+                    a[c][0]         = fma(c0, uc, a[c][0]);
+                    a[c][1]         = fma(c1, uc, a[c][1]);
+                    a[c][2]         = fma(c2, uc, a[c][2]);
+                  }
+              }
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+              for (int b = 0; b < 3; ++b)
+                F[(c * 3 + b) * m * n + qy * n + i] = a[c][b];
+          }
+        __syncthreads();
+        // integrate along y; accumulate the z integration of this level
+        if (r < n2)
+          {
+            const int i = r % n, j = r / n;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+              {
+                double xy = 0, z = 0;
+#pragma unroll 2
+                for (int qy = 0; qy < m; ++qy)
+                  {
+                    const double s  = sS[qy * n + j];
+                    const double g  = sG[qy * n + j];
+                    const int    o  = qy * n + i;
+                    xy = fma(s, F[(c * 3) * m * n + o], xy);
+                    xy = fma(g, F[(c * 3 + 1) * m * n + o], xy);
+                    z  = fma(s, F[(c * 3 + 2) * m * n + o], z);
+                  }
+#pragma unroll
+                for (int k = 0; k < n; ++k)
+                  acc[c][k] = fma(sS[qz * n + k], xy,
+                                  fma(sG[qz * n + k], z, acc[c][k]));
+              }
+          }
+      }
+
+    // ---- inverse mass with QGauss(k+1): S^-1 diag(1/JxW) S^-T, lines of
+    // the z-line owners (U is free) ----------------------------------------
+    const DevBasis<double> &B = *Bd;
+    // S^-T along z: (S^-T v)_q = sum_k Sinv[k][q] v_k
+    if (r < n2)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int q = 0; q < n; ++q)
+          {
+            double v = 0;
+#pragma unroll
+            for (int k = 0; k < n; ++k)
+              v = fma(B.Sinv[k][q], acc[c][k], v);
+            U[c * n3 + q * n2 + r] = v;
+          }
+    __syncthreads();
+    // S^-T along y: thread (i, qz)
+    if (r < n2)
+      {
+        const int i = r % n, qz = r / n;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          {
+            double v[n], o[n];
+#pragma unroll
+            for (int j = 0; j < n; ++j)
+              v[j] = U[c * n3 + qz * n2 + j * n + i];
+#pragma unroll
+            for (int q = 0; q < n; ++q)
+              {
+                double s = 0;
+#pragma unroll
+                for (int j = 0; j < n; ++j)
+                  s = fma(B.Sinv[j][q], v[j], s);
+                o[q] = s;
+              }
+#pragma unroll
+            for (int q = 0; q < n; ++q)
+              U[c * n3 + qz * n2 + q * n + i] = o[q];
+          }
+      }
+    __syncthreads();
+    // S^-T along x, 1/JxW, S^-1 along x: thread (qy, qz)
+    if (r < n2)
+      {
+        const int qy = r % n, qz = r / n;
+        double    ij[n];
+#pragma unroll
+        for (int qx = 0; qx < n; ++qx)
+          {
+            const double w = B.w[qx] * B.w[qy] * B.w[qz];
+            double       det;
+            if constexpr (cartesian)
+              det = P.h[0] * P.h[1] * P.h[2];
+            else
+              {
+                const double xi[3] = {B.xq[qx], B.xq[qy], B.xq[qz]};
+                double       x[3], J[3][3], adj[3][3];
+                multilinear_map<3>(X, xi, x, J);
+                det = adjugate<3>(J, adj);
+              }
+            ij[qx] = 1. / (det * w);
+          }
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          {
+            double v[n], g[n];
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+              v[i] = U[c * n3 + qz * n2 + qy * n + i];
+#pragma unroll
+            for (int q = 0; q < n; ++q)
+              {
+                double s = 0;
+#pragma unroll
+                for (int i = 0; i < n; ++i)
+                  s = fma(B.Sinv[i][q], v[i], s);
+                g[q] = s * ij[q];
+              }
+#pragma unroll
+            for (int i = 0; i < n; ++i)
+              {
+                double s = 0;
+#pragma unroll
+                for (int q = 0; q < n; ++q)
+                  s = fma(B.Sinv[i][q], g[q], s);
+                U[c * n3 + qz * n2 + qy * n + i] = s;
+              }
+          }
+      }
+    __syncthreads();
+    // S^-1 along y: thread (i, qz)
+    if (r < n2)
+      {
+        const int i = r % n, qz = r / n;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          {
+            double v[n], o[n];
+#pragma unroll
+            for (int q = 0; q < n; ++q)
+              v[q] = U[c * n3 + qz * n2 + q * n + i];
+#pragma unroll
+            for (int j = 0; j < n; ++j)
+              {
+                double s = 0;
+#pragma unroll
+                for (int q = 0; q < n; ++q)
+                  s = fma(B.Sinv[j][q], v[q], s);
+                o[j] = s;
+              }
+#pragma unroll
+            for (int j = 0; j < n; ++j)
+              U[c * n3 + qz * n2 + j * n + i] = o[j];
+          }
+      }
+    __syncthreads();
+    // S^-1 along z and the epilogue: thread (i, j) owns its z-line
+    if (r < n2 && active)
+      {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          {
+            double v[n];
+#pragma unroll
+            for (int q = 0; q < n; ++q)
+              v[q] = U[c * n3 + q * n2 + r];
+#pragma unroll
+            for (int k = 0; k < n; ++k)
+              {
+                double s = 0;
+#pragma unroll
+                for (int q = 0; q < n; ++q)
+                  s = fma(B.Sinv[k][q], v[q], s);
+                const int64_t o = cell * nvec + c * n3 + k * n2 + r;
+                if constexpr (epi == epi_store)
+                  io.dst[o] = io.scale * s;
+                else
+                  {
+                    const double s0 = io.sol_in[o];
+                    io.sol_out[o]   = fma(io.b_dt, s, s0);
+                    if (!io.last)
+                      io.ri_out[o] = fma(io.a_dt, s, s0);
+                  }
+              }
+          }
+      }
+  }
+} // namespace dgops

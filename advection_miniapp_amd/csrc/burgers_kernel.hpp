@@ -14,12 +14,15 @@
 
 namespace dgops
 {
+  // over-integration points per direction: k(k+1)/2 <= 28 at degree 7
+  constexpr int max_m = 28;
+
   struct BurgersTables
   {
-    int    m;            // over-integration points per direction
-    double Sm[16][max_n]; // nodal basis at the m Gauss points [q][i]
-    double Gm[16][max_n]; // its derivative
-    double wm[16], xm[16];
+    int    m;                // over-integration points per direction
+    double Sm[max_m][max_n]; // nodal basis at the m Gauss points [q][i]
+    double Gm[max_m][max_n]; // its derivative
+    double wm[max_m], xm[max_m];
   };
 
   template <int dim>

@@ -462,7 +462,8 @@ dg_create(const dg_config *cfg, void *stream, dg_operator **out)
     if (cfg->operator_kind == DG_OP_BURGERS)
       {
         // QGauss(k(k+1)/2) tables (burgers_solver.cc:234)
-        require(cfg->degree <= 4, "Burgers operator: degree 1..4",
+        require(cfg->degree <= (cfg->dim == 3 ? 7 : 4),
+                "Burgers operator: degree 1..7 in 3D, 1..4 in 2D",
                 DG_ERR_UNSUPPORTED);
         require(cfg->precision == DG_FP64, "Burgers operator: FP64",
                 DG_ERR_UNSUPPORTED);
@@ -567,8 +568,12 @@ dg_apply(dg_operator *h, const void *src, void *dst, double t)
         launch_burgers_op(*op, P, epi_store, &io);
         return;
       }
-    require(op->cfg.factor_skew == 0.,
-            "explicit operator is built for factor_skew = 0",
+    // skew-symmetric form (factor_skew, advection_solver.cc:106-108,
+    // :500-507): built for advection; SIPG uses the conservative form
+    const bool skew = op->cfg.factor_skew != 0.;
+    require(!skew || op->cfg.operator_kind == DG_OP_ADVECTION,
+            "explicit advection-diffusion operator is built for "
+            "factor_skew = 0",
             DG_ERR_UNSUPPORTED);
     OpParams P = op->base_params(t, 1);
     if (op->fp32())
@@ -577,7 +582,7 @@ dg_apply(dg_operator *h, const void *src, void *dst, double t)
         io.src   = (const float *)src;
         io.dst   = (float *)dst;
         io.scale = 1.f;
-        launch_operator(*op, P, 1, true, false, fin_inverse_mass, epi_store,
+        launch_operator(*op, P, 1, true, skew, fin_inverse_mass, epi_store,
                         &io);
       }
     else
@@ -586,7 +591,7 @@ dg_apply(dg_operator *h, const void *src, void *dst, double t)
         io.src   = (const double *)src;
         io.dst   = (double *)dst;
         io.scale = 1.;
-        launch_operator(*op, P, 1, false, false, fin_inverse_mass, epi_store,
+        launch_operator(*op, P, 1, false, skew, fin_inverse_mass, epi_store,
                         &io);
       }
   });
@@ -616,8 +621,10 @@ namespace dgops
         launch_burgers_op(op, P, epi_lsrk, &io);
         return;
       }
-    require(op.cfg.factor_skew == 0.,
-            "explicit operator is built for factor_skew = 0",
+    const bool skew = op.cfg.factor_skew != 0.;
+    require(!skew || op.cfg.operator_kind == DG_OP_ADVECTION,
+            "explicit advection-diffusion operator is built for "
+            "factor_skew = 0",
             DG_ERR_UNSUPPORTED);
     // the LSRK stage accumulates in FP64; FP32 operators give right-hand
     // sides (apply) only
@@ -634,7 +641,7 @@ namespace dgops
         io.b_dt    = b_dt;
         io.a_dt    = a_dt;
         io.last    = last;
-        launch_operator(op, P, 1, false, false, fin_inverse_mass, epi_lsrk,
+        launch_operator(op, P, 1, false, skew, fin_inverse_mass, epi_lsrk,
                         &io);
       }
   }

@@ -22,13 +22,16 @@ SIPG_GEO = (GEO_ANALYTIC, GEO_UNIFORM)
 
 def op_variants():
     v = []
-    # explicit right-hand side M^-1(-L u): apply and fused LSRK stage
-    for dim, n, cart, epi in itertools.product(
-            (2, 3), range(2, 9), ADVECTION_GEO, (EPI_STORE, EPI_LSRK)):
-        v.append((dim, n, 1, 0, cart, 0, 0, FIN_INVMASS, epi))
+    # explicit right-hand side M^-1(-L u): apply and fused LSRK stage, the
+    # conservative (skew 0) and the skew-symmetric form
+    for dim, n, cart, epi, skew in itertools.product(
+            (2, 3), range(2, 9), ADVECTION_GEO, (EPI_STORE, EPI_LSRK),
+            (0, 1)):
+        v.append((dim, n, 1, 0, cart, skew, 0, FIN_INVMASS, epi))
     # fp32 explicit right-hand side
-    for dim, n, cart in itertools.product((2, 3), range(2, 9), ADVECTION_GEO):
-        v.append((dim, n, 1, 1, cart, 0, 0, FIN_INVMASS, EPI_STORE))
+    for dim, n, cart, skew in itertools.product((2, 3), range(2, 9),
+                                                ADVECTION_GEO, (0, 1)):
+        v.append((dim, n, 1, 1, cart, skew, 0, FIN_INVMASS, EPI_STORE))
     # implicit single-stage operator (M/dt + L) and IRK right-hand side,
     # skew-symmetric form, fp64 and fp32
     for dim, n, cart, fp32 in itertools.product(
@@ -58,11 +61,11 @@ N_SHAPES = 3
 
 def tile_shapes(var):
     """Tile shapes built for an operator variant: all three for the 3D
-    explicit right-hand sides (apply and fused LSRK stage, advection and
-    SIPG, FP64 and FP32), which the autotuner chooses among; the default
-    shape for the others."""
+    explicit right-hand sides in conservative form (apply and fused LSRK
+    stage, advection and SIPG, FP64 and FP32), which the autotuner chooses
+    among; the default shape for the others."""
     dim, n, nc, fp32, geo, skew, diff, fin, epi = var
-    if dim == 3 and nc == 1 and fin == FIN_INVMASS:
+    if dim == 3 and nc == 1 and fin == FIN_INVMASS and not skew:
         return range(N_SHAPES)
     return (0,)
 
@@ -89,9 +92,13 @@ FDM_TYPES = {0: ("double", "double", "double"), 1: ("float", "float", "float"),
 
 
 def burgers_variants():
+    # 2D: the point-wise LDS kernel, degrees 1..4; 3D: the slab-wise
+    # kernel, degrees 1..7
     v = []
-    for dim, k, cart, epi in itertools.product((2, 3), (1, 2, 3, 4), (0, 1),
+    for dim, k, cart, epi in itertools.product((2, 3), range(1, 8), (0, 1),
                                                (EPI_STORE, EPI_LSRK)):
+        if dim == 2 and k > 4:
+            continue
         v.append(("burgers", dim, k + 1, k * (k + 1) // 2, cart, epi))
     return v
 
@@ -156,9 +163,10 @@ def emit(path, variants):
             continue
         if var[0] == "burgers":
             _, dim, n, m, cart, epi = var
+            fn = "launch_burgers3" if dim == 3 else "launch_burgers"
             lines.append(
                 f"  static BurgersRegistrar g{i}(burgers_key({dim}, {n}, {cart}, "
-                f"{epi}), &launch_burgers<{dim}, {n}, {m}, {b(cart)}, {epi}>);")
+                f"{epi}), &{fn}<{dim}, {n}, {m}, {b(cart)}, {epi}>);")
             continue
         if var[0] == "fdm":
             _, dim, n, ns, prec = var
@@ -189,7 +197,7 @@ def main(n_files=24):
         if v[0] == "fdm":
             return v[2] ** (v[1] + 1) * v[3]
         if v[0] == "burgers":
-            return v[3] ** (v[1] + 1)
+            return v[3] ** v[1] * v[2]
         if v[0] in ("batch", "mass", "colloc", "cellgmres"):
             return v[2] ** (v[1] + 1)
         return v[1] ** (v[0] + 1) * v[2] * (2 if v[6] else 1)

@@ -578,6 +578,12 @@ namespace dgops
     __shared__ T      s_face[dim == 3 ? CPB * FSTR : 1];
     __shared__ T      s_fan[AN ? CPB * 2 * (dim - 1) * nl : 1];
     __shared__ double s_X[CPB * NX];
+    // several components (IRK stages): their values at the quadrature
+    // points wait in LDS between the value phase and the operator phase
+    // (LAST-line layout: every thread writes and reads its own line only)
+    // instead of holding NC lines in registers through the operator phase
+    constexpr bool STAGE_LDS = NC > 1;
+    __shared__ T   s_stage[STAGE_LDS ? NC * CPB * STR : 1];
     const T *tab_c = static_cast<const T *>(P.tab_cell);
     const T *tab_f = static_cast<const T *>(P.tab_face);
 
@@ -598,6 +604,9 @@ namespace dgops
     T      *f_l   = dim == 3 ? s_face + lcs * FSTR : s_face;
     T      *fan_l = AN ? s_fan + lcs * 2 * (dim - 1) * nl : s_fan;
     double *X     = s_X + lcs * NX;
+    auto    stage_l = [&](int c) -> T * {
+      return s_stage + (STAGE_LDS ? (c * CPB + lcs) * STR : 0);
+    };
 
     // the cell's own values of the first component are requested first:
     // the x sweep then overlaps the neighbour-index -> trace load chain
@@ -678,7 +687,25 @@ namespace dgops
     // ---- per component: neighbour traces and values at quadrature points
     T uplus[NC][2 * dim];
     T dplus[NC][2 * dim];
-    T uqs[NC][n];
+    // values at the quadrature points on this thread's LAST line: in
+    // registers for one component, in s_stage for several
+    T    uqs[STAGE_LDS ? 1 : NC][n];
+    auto keep_uq = [&](int c, const T(&v)[n]) {
+      if constexpr (STAGE_LDS)
+        lds_write_line<dim, n, LAST>(stage_l(c), p, v);
+      else
+#pragma unroll
+        for (int i = 0; i < n; ++i)
+          uqs[0][i] = v[i];
+    };
+    auto get_uq = [&](int c, T(&v)[n]) {
+      if constexpr (STAGE_LDS)
+        lds_read_line<dim, n, LAST>(stage_l(c), p, v);
+      else
+#pragma unroll
+        for (int i = 0; i < n; ++i)
+          v[i] = uqs[0][i];
+    };
 #pragma unroll
     for (int c = 0; c < NC; ++c)
       {
@@ -693,9 +720,7 @@ namespace dgops
         if constexpr (fin == fin_local)
           {
             // already at the Gauss points; the neighbours do not couple
-#pragma unroll
-            for (int i = 0; i < n; ++i)
-              uqs[c][i] = r[i];
+            keep_uq(c, r);
 #pragma unroll
             for (int f = 0; f < 2 * dim; ++f)
               uplus[c][f] = dplus[c][f] = 0;
@@ -900,9 +925,7 @@ namespace dgops
             lds_read_line<dim, n, 2>(acc_l, p, t);
             eo_apply<n, false, false, false>(B.eo_S, t, r);
           }
-#pragma unroll
-        for (int i = 0; i < n; ++i)
-          uqs[c][i] = r[i];
+        keep_uq(c, r);
         __syncthreads();
       }
 
@@ -1037,28 +1060,32 @@ namespace dgops
               for (int b = 0; b < NC; ++b)
                 {
                   const T m = (T)P.mass[c][b];
+                  T       ub[n];
+                  get_uq(b, ub);
 #pragma unroll
                   for (int i = 0; i < n; ++i)
-                    acc[i] = fma(m * jxw[i], uqs[b][i], acc[i]);
+                    acc[i] = fma(m * jxw[i], ub[i], acc[i]);
                 }
             }
+          T uc[n];
+          get_uq(c, uc);
           T g[n];
 #pragma unroll
           for (int i = 0; i < n; ++i)
-            g[i] = fcell * cv[LAST][i] * uqs[c][i];
+            g[i] = fcell * cv[LAST][i] * uc[i];
           eo_apply<n, true, true, true>(B.eo_D, g, acc);
           if constexpr (skew)
             {
               T du[n];
-              eo_apply<n, true, false, false>(B.eo_D, uqs[c], du);
+              eo_apply<n, true, false, false>(B.eo_D, uc, du);
 #pragma unroll
               for (int i = 0; i < n; ++i)
                 acc[i] = fma(fs * cv[LAST][i], du[i], acc[i]);
             }
           if constexpr (diff)
-            diffusion_cell_term<dim, n, LAST, T>(P, Bd, B, p, uqs[c], acc);
+            diffusion_cell_term<dim, n, LAST, T>(P, Bd, B, p, uc, acc);
           face_terms<dim, n, LAST, T, diff, fin == fin_local>(P, Bd, B, cell, p, active, nb,
-                                            uplus[c], dplus[c], uqs[c], acc,
+                                            uplus[c], dplus[c], uc, acc,
                                             vf, fan);
           if constexpr (early_ij)
             {
@@ -1070,19 +1097,19 @@ namespace dgops
               table_line<dim, n>(tab_c, cell, dim, p, active, ij_row);
               __builtin_amdgcn_s_setprio(0);
             }
-          lds_write_line<dim, n, LAST>(u_l, p, uqs[c]);
+          lds_write_line<dim, n, LAST>(u_l, p, uc);
           lds_write_line<dim, n, LAST>(acc_l, p, acc);
           if constexpr (AN)
             {
 #pragma unroll
               for (int i = 0; i < n; ++i)
-                g[i] = fcell * cv[0][i] * uqs[c][i];
+                g[i] = fcell * cv[0][i] * uc[i];
               lds_write_line<dim, n, LAST>(g0_l, p, g);
               if constexpr (dim == 3)
                 {
 #pragma unroll
                   for (int i = 0; i < n; ++i)
-                    g[i] = fcell * cv[1][i] * uqs[c][i];
+                    g[i] = fcell * cv[1][i] * uc[i];
                   lds_write_line<dim, n, LAST>(g1_l, p, g);
                 }
               if constexpr (skew)

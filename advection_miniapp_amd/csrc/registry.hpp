@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <map>
 
+#include "burgers3d_kernel.hpp"
 #include "burgers_kernel.hpp"
 #include "colloc_kernel.hpp"
 #include "fdm_kernel.hpp"
@@ -128,6 +129,30 @@ namespace dgops
                        0,
                        stream,
                        P,
+                       (const DevBasis<double> *)bd,
+                       (const BurgersTables *)tables,
+                       *(const OpBuffers<double> *)io);
+    return hipGetLastError();
+  }
+
+  // 3D Burgers, degrees 1..7 (slab-wise cell term)
+  template <int dim, int n, int m, bool cart, int epi>
+  hipError_t
+  launch_burgers3(const OpParams &P,
+                  const void     *bd,
+                  const void     *tables,
+                  const void     *io,
+                  hipStream_t     stream)
+  {
+    static_assert(dim == 3, "the slab-wise Burgers kernel is 3D");
+    constexpr int CPB = burgers3_cells(n, m);
+    constexpr int R   = burgers3_threads(n, m);
+    static_assert(CPB * R <= 1024, "workgroup size");
+    const int64_t blocks = (P.n_cells + CPB - 1) / CPB;
+    if (blocks <= 0)
+      return hipSuccess;
+    hipLaunchKernelGGL((burgers3_kernel<n, m, cart, epi, CPB>),
+                       dim3((unsigned)blocks), dim3(CPB * R), 0, stream, P,
                        (const DevBasis<double> *)bd,
                        (const BurgersTables *)tables,
                        *(const OpBuffers<double> *)io);

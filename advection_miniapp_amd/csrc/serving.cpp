@@ -3,11 +3,13 @@
 // advanced together one time step per dg_server_step. All active solves of
 // one degree share one kernel launch per LSRK stage (the batched operator
 // kernel finds its solve from the workgroup index), so small solves fill
-// the GPU together and a solve's arithmetic is the one it gets alone.
+// the GPU together and a solve's arithmetic is the one it gets alone; the
+// launches of different degrees run on streams of their own and overlap.
 // Solves join and leave at step boundaries (continuous batching).
 #include <algorithm>
 #include <cmath>
 #include <deque>
+#include <iterator>
 #include <map>
 #include <memory>
 #include <string>
@@ -140,10 +142,43 @@ namespace dgops
     BatchItem<double>                           *h_items = nullptr;
     size_t                                       h_cap   = 0;
 
+    // one stream per degree group (at most max_lanes): the launches of
+    // different degrees overlap on the device; each degree's stages stay in
+    // order on its stream. Default priority, joined to `stream` by events.
+    static constexpr int    max_lanes = 4;
+    std::vector<hipStream_t> lanes;
+    std::vector<hipEvent_t>  lane_done;
+    hipEvent_t               ready = nullptr;
+
     ~Server()
     {
       if (h_items)
         (void)hipHostFree(h_items);
+      for (hipStream_t l : lanes)
+        (void)hipStreamDestroy(l);
+      for (hipEvent_t e : lane_done)
+        (void)hipEventDestroy(e);
+      if (ready)
+        (void)hipEventDestroy(ready);
+    }
+
+    void
+    make_lanes(int n)
+    {
+      if (!ready)
+        hip_check(hipEventCreateWithFlags(&ready, hipEventDisableTiming),
+                  "serving event");
+      while ((int)lanes.size() < n)
+        {
+          hipStream_t l;
+          hipEvent_t  e;
+          hip_check(hipStreamCreateWithFlags(&l, hipStreamNonBlocking),
+                    "serving stream");
+          lanes.push_back(l);
+          hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming),
+                    "serving event");
+          lane_done.push_back(e);
+        }
     }
 
     // device memory of a request: three registers, mesh arrays, bases
@@ -271,13 +306,19 @@ namespace dgops
         int         count;
         unsigned    blocks;
         const void *basis;
+        int         lane;
       };
+      const int n_lanes = std::min<int>(max_lanes, (int)groups.size());
+      make_lanes(n_lanes);
       std::vector<Launch> launches;
       size_t              k = 0;
       for (int j = 0; j < n_run; ++j)
         for (const LsrkStage &s : plan)
           for (auto &g : groups)
             {
+              const int lane = (int)std::distance(groups.begin(),
+                                                  groups.find(g.first)) %
+                               n_lanes;
               const int n = g.first;
               // the tile shape tuned for the group's largest solve
               const ServeReq *big = g.second.front();
@@ -291,7 +332,8 @@ namespace dgops
                       DG_ERR_UNSUPPORTED);
               const int64_t cpb = tile_cells(3, n, big->shape);
               Launch        L   = {it->second, k, 0, 0,
-                                   g.second.front()->op()->d_basis_d.ptr};
+                                   g.second.front()->op()->d_basis_d.ptr,
+                                   lane};
               unsigned      first = 0;
               for (ServeReq *q : g.second)
                 {
@@ -318,9 +360,20 @@ namespace dgops
                                hipMemcpyHostToDevice, stream),
                 "upload descriptors");
       const auto *dev = d_items.as<BatchItem<double>>();
+      // the lanes start after the descriptors (and earlier admissions)
+      hip_check(hipEventRecord(ready, stream), "serving event");
+      for (int l = 0; l < n_lanes; ++l)
+        hip_check(hipStreamWaitEvent(lanes[l], ready, 0), "serving wait");
       for (const Launch &L : launches)
-        hip_check(L.f(dev + L.first, L.count, L.blocks, L.basis, stream),
+        hip_check(L.f(dev + L.first, L.count, L.blocks, L.basis,
+                      lanes[L.lane]),
                   "batched operator launch");
+      for (int l = 0; l < n_lanes; ++l)
+        {
+          hip_check(hipEventRecord(lane_done[l], lanes[l]), "serving event");
+          hip_check(hipStreamWaitEvent(stream, lane_done[l], 0),
+                    "serving wait");
+        }
       // the pinned descriptors are reused by the next call
       hip_check(hipStreamSynchronize(stream), "serving step");
       // retire finished solves: keep the solution register only

@@ -26,7 +26,9 @@ def burgers(dim, k, refine, **kw):
 
 
 @pytest.mark.parametrize("dim,k,pert", [(3, 3, 0.0), (3, 2, 0.1), (2, 4, 0.0),
-                                        (2, 1, 0.15), (3, 4, 0.0)])
+                                        (2, 1, 0.15), (3, 4, 0.0),
+                                        (3, 1, 0.0), (3, 5, 0.1), (3, 6, 0.0),
+                                        (3, 7, 0.0), (3, 7, 0.1)])
 def test_burgers_apply_matches_oracle(gpu_lib, dim, k, pert):
     c = burgers(dim, k, 1 if dim == 3 else 2, perturbation=pert)
     op, o = AdvectionOperation(c.dg()), c.oracle_handle()

@@ -44,6 +44,18 @@ CASES = {
     "3d_k7_cart": dict(dim=3, degree=7, refine=1, velocity=[-1, 0.5, 0.25]),
     "2d_k7_pert": dict(dim=2, degree=7, refine=2, perturbation=0.1,
                        velocity_kind=3, time_factor=1, final_time=8.0),
+    # skew-symmetric form of the explicit operator (factor_skew,
+    # advection_solver.cc:106-108)
+    "3d_k3_pert_skew": dict(dim=3, degree=3, refine=2, perturbation=0.1,
+                            velocity_kind=1, factor_skew=0.5),
+    "2d_k4_vortex_skew": dict(dim=2, degree=4, refine=3, perturbation=0.1,
+                              velocity_kind=2, time_factor=1,
+                              final_time=8.0, factor_skew=0.3,
+                              flux_alpha=0.0),
+    "3d_k2_dirichlet_skew": dict(dim=3, degree=2, n_coarse=[2, 1, 1],
+                                 refine=1, periodic=[0, 1, 0],
+                                 velocity=[1.1, 0.15, -0.05],
+                                 solution_kind=1, factor_skew=0.5),
 }
 
 
@@ -144,7 +156,8 @@ def test_apply_perturbed_mesh_transport_converges(gpu_lib, velocity_kind):
 
 
 @pytest.mark.parametrize("name", ["3d_k4_pert_cellular", "2d_k4_pert_vortex",
-                                  "2d_k2_dirichlet"])
+                                  "2d_k2_dirichlet", "3d_k3_pert_skew",
+                                  "3d_k2_dirichlet_skew"])
 def test_perform_stage_matches_oracle(gpu_lib, name):
     c, op, o = make(name)
     rng = np.random.default_rng(3)

@@ -348,14 +348,17 @@ def _burgers(dim=3, k=3, refine=1, **kw):
                solution_kind=6, **kw)
 
 
-def test_burgers_constant_state_is_steady():
-    o = _burgers(perturbation=0.1).oracle_handle()
+@pytest.mark.parametrize("k", [3, 6])
+def test_burgers_constant_state_is_steady(k):
+    o = _burgers(k=k, perturbation=0.1).oracle_handle()
     u = np.zeros(o.n_dofs * 3)
-    nd = 64
+    nd = (k + 1) ** 3
     for c, val in enumerate((0.7, -0.3, 1.1)):
         u.reshape(-1, 3, nd)[:, c, :] = val
     du = o.burgers_apply(u)
-    assert np.max(np.abs(du)) < 1e-12
+    # round-off of the inverse mass grows with k (2e-13 at k = 3, 1.5e-12
+    # at k = 6); an operator that does not keep the state steady gives O(1)
+    assert np.max(np.abs(du)) < 1e-11
 
 
 def test_burgers_smooth_right_hand_side():
