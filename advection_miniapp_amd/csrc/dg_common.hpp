@@ -47,6 +47,10 @@ namespace dgops
     double        sol_param[8];
     double        viscosity;
     double        penalty;
+    // >= 0: the mesh is one periodic 3D coarse cell refined morton_bits
+    // times, so the cell index is the Morton code of (i, j, k) (x in bit
+    // 3b) and the face neighbours follow from it; -1: read `neighbors`
+    int           morton_bits;
     // geometry tables of the tabulated-geometry kernels (scalar type of
     // the kernel): per cell [dim + 1][nd] = w adj(J) a (dim rows) and
     // 1/JxW at the Gauss points, each row in the line order of the

@@ -268,6 +268,12 @@ namespace dgops
       P.sol_param[i] = cfg.solution_param[i];
     P.viscosity = cfg.viscosity;
     P.penalty   = cfg.penalty_factor;
+    // one periodic 3D coarse cell: neighbours from the Morton code
+    const bool one_periodic_cube =
+      cfg.dim == 3 && cfg.refine <= 10 && cfg.n_coarse[0] == 1 &&
+      cfg.n_coarse[1] == 1 && cfg.n_coarse[2] == 1 && cfg.periodic[0] &&
+      cfg.periodic[1] && cfg.periodic[2];
+    P.morton_bits = one_periodic_cube ? cfg.refine : -1;
     return P;
   }
 

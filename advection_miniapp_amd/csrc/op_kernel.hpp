@@ -635,9 +635,27 @@ namespace dgops
             X[e] = P.cell_vertices[cell * NV * dim + e];
       }
     int nb[2 * dim];
+    if (dim == 3 && P.morton_bits >= 0)
+      {
+        // one periodic coarse cell: the neighbours are Morton increments
+        // of the cell index (dilated integers, carries confined to one
+        // coordinate's bits), so the trace loads below need no index load
+        // first
+        const unsigned full = (1u << (3 * P.morton_bits)) - 1u;
+        const unsigned m    = (unsigned)cell;
 #pragma unroll
-    for (int f = 0; f < 2 * dim; ++f)
-      nb[f] = active ? P.neighbors[cell * 2 * dim + f] : -1;
+        for (int d = 0; d < dim; ++d)
+          {
+            const unsigned M = (0x09249249u << d) & full;
+            nb[2 * d] = active ? (int)((((m & M) - 1u) & M) | (m & ~M)) : -1;
+            nb[2 * d + 1] =
+              active ? (int)((((m | ~M) + 1u) & M) | (m & ~M)) : -1;
+          }
+      }
+    else
+#pragma unroll
+      for (int f = 0; f < 2 * dim; ++f)
+        nb[f] = active ? P.neighbors[cell * 2 * dim + f] : -1;
     if constexpr (needX)
       __syncthreads();
 
@@ -1284,8 +1302,9 @@ namespace dgops
               {
 #pragma unroll
                 for (int i = 0; i < n; ++i)
-                  io.dst[base + L::template off<LAST>(p, i)] =
-                    io.scale * acc[i];
+                  __builtin_nontemporal_store(
+                    io.scale * acc[i],
+                    &io.dst[base + L::template off<LAST>(p, i)]);
               }
             else
               {
@@ -1294,9 +1313,11 @@ namespace dgops
                   {
                     const int64_t k  = base + L::template off<LAST>(p, i);
                     const T       s0 = io.sol_in[k];
-                    io.sol_out[k]    = fma(io.b_dt, acc[i], s0);
+                    __builtin_nontemporal_store(fma(io.b_dt, acc[i], s0),
+                                                &io.sol_out[k]);
                     if (!io.last)
-                      io.ri_out[k] = fma(io.a_dt, acc[i], s0);
+                      __builtin_nontemporal_store(fma(io.a_dt, acc[i], s0),
+                                                  &io.ri_out[k]);
                   }
               }
           }
