@@ -584,6 +584,13 @@ namespace dgops
     // instead of holding NC lines in registers through the operator phase
     constexpr bool STAGE_LDS = NC > 1;
     __shared__ T   s_stage[STAGE_LDS ? NC * CPB * STR : 1];
+    // several components, 3D advection: the neighbour traces of component
+    // c are interpolated at the start of its operator phase instead of in
+    // the value phase, so that one component's traces are live at a time
+    // (NC sets of traces held through the operator phase cost one
+    // occupancy step)
+    constexpr bool TRACE_LATE =
+      STAGE_LDS && dim == 3 && !diff && fin != fin_local;
     const T *tab_c = static_cast<const T *>(P.tab_cell);
     const T *tab_f = static_cast<const T *>(P.tab_face);
 
@@ -703,8 +710,9 @@ namespace dgops
     };
 
     // ---- per component: neighbour traces and values at quadrature points
-    T uplus[NC][2 * dim];
-    T dplus[NC][2 * dim];
+    constexpr int NUP = TRACE_LATE ? 1 : NC;
+    T             uplus[NUP][2 * dim];
+    T             dplus[NUP][2 * dim];
     // values at the quadrature points on this thread's LAST line: in
     // registers for one component, in s_stage for several
     T    uqs[STAGE_LDS ? 1 : NC][n];
@@ -748,7 +756,7 @@ namespace dgops
         // (t1 = i, t2 = j) of every face, one load per face); stored after
         // the x sweep so that the sweep does not wait for them
         T tv[6];
-        if constexpr (dim == 3 && !diff)
+        if constexpr (dim == 3 && !diff && !TRACE_LATE)
           {
             const int i0 = p % n, j0 = p / n;
 #pragma unroll
@@ -774,10 +782,13 @@ namespace dgops
         lds_write_line<dim, n, 0>(acc_l, p, t);
         if constexpr (dim == 3 && !diff)
           {
-            const int i0 = p % n, j0 = p / n;
+            if constexpr (!TRACE_LATE)
+              {
+                const int i0 = p % n, j0 = p / n;
 #pragma unroll
-            for (int f = 0; f < 6; ++f)
-              f_l[f * nl + j0 * n + i0] = tv[f];
+                for (int f = 0; f < 6; ++f)
+                  f_l[f * nl + j0 * n + i0] = tv[f];
+              }
           }
         else if constexpr (dim == 3)
           {
@@ -842,6 +853,8 @@ namespace dgops
         T fv[6];
         if constexpr (dim == 3 && !diff)
           {
+            if constexpr (!TRACE_LATE)
+            {
             // first tangential interpolation of the staged nodes: thread
             // (aq, j); written back after the next barrier
             const int aq = p % n, j = p / n;
@@ -854,6 +867,7 @@ namespace dgops
                   v = fma(B.S[aq][i], f_l[f * nl + j * n + i], v);
                 fv[f] = v;
               }
+            }
           }
         else if constexpr (dim == 3)
           {
@@ -917,7 +931,7 @@ namespace dgops
         if constexpr (dim == 3)
           {
             __syncthreads();
-            if constexpr (!diff)
+            if constexpr (!diff && !TRACE_LATE)
               {
                 const int aq = p % n, j = p / n;
 #pragma unroll
@@ -926,7 +940,7 @@ namespace dgops
               }
             lds_write_line<dim, n, 1>(acc_l, p, r);
             __syncthreads();
-            if constexpr (!diff)
+            if constexpr (!diff && !TRACE_LATE)
               {
                 // second tangential interpolation: thread (aq, bq)
                 const int aq = p % n, bq = p / n;
@@ -1063,6 +1077,60 @@ namespace dgops
 #pragma unroll
     for (int c = 0; c < NC; ++c)
       {
+        if constexpr (TRACE_LATE)
+          {
+            // neighbour traces of component c: thread (i0, j0) stages node
+            // (t1 = i0, t2 = j0) of each face, then interpolates along t1
+            // (thread (aq = i0, j0)) and along t2 (thread (aq = i0, bq = j0))
+            const T  *src = io.src + c * io.comp_stride;
+            const int i0 = p % n, j0 = p / n;
+            T         tv[6];
+            __builtin_amdgcn_s_setprio(3);
+#pragma unroll
+            for (int f = 0; f < 6; ++f)
+              {
+                const int d = f / 2, s = f % 2;
+                const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+                tv[f]        = 0;
+                if (active && nb[f] >= 0)
+                  {
+                    int ijk[3];
+                    ijk[d]  = s ? 0 : n - 1;
+                    ijk[t1] = i0;
+                    ijk[t2] = j0;
+                    tv[f]   = src[(int64_t)nb[f] * nd + ijk[0] + n * ijk[1] +
+                                n * n * ijk[2]];
+                  }
+              }
+            __builtin_amdgcn_s_setprio(0);
+#pragma unroll
+            for (int f = 0; f < 6; ++f)
+              f_l[f * nl + j0 * n + i0] = tv[f];
+            __syncthreads();
+#pragma unroll
+            for (int f = 0; f < 6; ++f)
+              {
+                T v = 0;
+#pragma unroll
+                for (int i = 0; i < n; ++i)
+                  v = fma(B.S[i0][i], f_l[f * nl + j0 * n + i], v);
+                tv[f] = v;
+              }
+            __syncthreads();
+#pragma unroll
+            for (int f = 0; f < 6; ++f)
+              f_l[f * nl + j0 * n + i0] = tv[f];
+            __syncthreads();
+#pragma unroll
+            for (int f = 0; f < 6; ++f)
+              {
+                T v = 0;
+#pragma unroll
+                for (int jj = 0; jj < n; ++jj)
+                  v = fma(B.S[j0][jj], f_l[f * nl + jj * n + i0], v);
+                uplus[0][f] = v;
+              }
+          }
         const T vf    = (T)P.vel_factor[c];
         const T fcell = (T)(-(1. - P.skew) * P.sign) * vf;
         const T fs    = (T)(P.skew * P.sign) * vf;
@@ -1103,7 +1171,8 @@ namespace dgops
           if constexpr (diff)
             diffusion_cell_term<dim, n, LAST, T>(P, Bd, B, p, uc, acc);
           face_terms<dim, n, LAST, T, diff, fin == fin_local>(P, Bd, B, cell, p, active, nb,
-                                            uplus[c], dplus[c], uc, acc,
+                                            uplus[NUP > 1 ? c : 0],
+                                            dplus[NUP > 1 ? c : 0], uc, acc,
                                             vf, fan);
           if constexpr (early_ij)
             {
@@ -1181,7 +1250,8 @@ namespace dgops
             if constexpr (diff)
               diffusion_cell_term<dim, n, 1, T>(P, Bd, B, p, uq, acc);
             face_terms<dim, n, 1, T, diff, fin == fin_local>(P, Bd, B, cell, p, active, nb,
-                                           uplus[c], dplus[c], uq, acc, vf,
+                                           uplus[NUP > 1 ? c : 0],
+                                           dplus[NUP > 1 ? c : 0], uq, acc, vf,
                                            fan);
             lds_write_line<dim, n, 1>(acc_l, p, acc);
             __syncthreads();
@@ -1219,7 +1289,8 @@ namespace dgops
           if constexpr (diff)
             diffusion_cell_term<dim, n, 0, T>(P, Bd, B, p, uq, acc);
           face_terms<dim, n, 0, T, diff, fin == fin_local>(P, Bd, B, cell, p, active, nb,
-                                         uplus[c], dplus[c], uq, acc, vf,
+                                         uplus[NUP > 1 ? c : 0],
+                                         dplus[NUP > 1 ? c : 0], uq, acc, vf,
                                          fan);
         }
         __syncthreads();
