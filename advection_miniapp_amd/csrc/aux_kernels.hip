@@ -1068,8 +1068,8 @@ namespace dgops
     auto      it   = mass_registry().find(mass_key(op.cfg.dim, op.n, fp32));
     if (it == mass_registry().end())
       throw Error(DG_ERR_UNSUPPORTED, "inverse mass variant not built");
-    hip_check(it->second(P, op.basis_ptr(fp32 != 0), op.d_basis_d.ptr, vec,
-                         op.stream),
+    hip_check(it->second(P, op.mass_basis_ptr(fp32 != 0),
+                         op.mass_basis_ptr(false), vec, op.stream),
               "inverse mass");
   }
 

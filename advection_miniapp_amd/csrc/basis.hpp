@@ -4,7 +4,9 @@
 //
 // The element is FE_DGQ(k) of deal.II: Lagrange polynomials on the k+1
 // Gauss-Lobatto nodes, lexicographic numbering (advection_solver.cc:961,
-// `FE_DGQ<dim> fe`), integrated with QGauss(k+1) (advection_solver.cc:440-445).
+// `FE_DGQ<dim> fe`), integrated with QGauss(k+1), or with
+// QGaussLobatto(k+1) under use_gl_quad (advection_solver.cc:110-116,
+// :440-445).
 #pragma once
 
 #include <cmath>
@@ -145,11 +147,33 @@ namespace dgops
       }
   }
 
+  // Gauss-Lobatto weights on [0,1]: 1 / (n (n-1) P_{n-1}(x_i)^2) at the
+  // points of gauss_lobatto_points_ext (x_i mapped to [-1,1])
+  inline std::vector<real_ext>
+  gauss_lobatto_weights_ext(int n)
+  {
+    std::vector<real_ext> w(n, (real_ext)1);
+    if (n == 1)
+      return w;
+    const auto x = gauss_lobatto_points_ext(n);
+    for (int i = 0; i < n; ++i)
+      {
+        real_ext p, dp, ddp;
+        legendre(n - 1, 2 * x[i] - 1, p, dp, ddp);
+        w[i] = 1 / ((real_ext)n * (n - 1) * p * p);
+      }
+    return w;
+  }
+
   // All 1D data one degree needs. Matrices are row-major [row][col] with
-  // the row index first, padded to max_n.
+  // the row index first, padded to max_n. The quadrature ("Gauss") points
+  // and weights are Gauss-Legendre, or Gauss-Lobatto for the
+  // use_gl_quad switch: then they coincide with the nodes, S and S^-1 are
+  // the identity and l0, l1 pick the end nodes.
   struct Basis1D
   {
     int    n;
+    bool   gl_quadrature;
     double gauss_pts[max_n];
     double gauss_wts[max_n];
     double gl_pts[max_n];
@@ -168,16 +192,28 @@ namespace dgops
   };
 
   inline Basis1D
-  make_basis(int n)
+  make_basis(int n, bool gl_quadrature = false)
   {
     if (n < 1 || n > max_n)
       throw std::runtime_error("unsupported number of 1D points");
+    if (gl_quadrature && n < 2)
+      throw std::runtime_error("Gauss-Lobatto quadrature needs degree >= 1");
     Basis1D b = {};
-    b.n = n;
+    b.n             = n;
+    b.gl_quadrature = gl_quadrature;
     std::vector<double> gp, gw;
     gauss_points(n, gp, gw);
-    const auto xg = gauss_points_ext(n);
     const auto xl = gauss_lobatto_points_ext(n);
+    const auto xg = gl_quadrature ? xl : gauss_points_ext(n);
+    if (gl_quadrature)
+      {
+        const auto wl = gauss_lobatto_weights_ext(n);
+        for (int q = 0; q < n; ++q)
+          {
+            gp[q] = (double)xl[q];
+            gw[q] = (double)wl[q];
+          }
+      }
     for (int q = 0; q < n; ++q)
       {
         b.gauss_pts[q] = gp[q];

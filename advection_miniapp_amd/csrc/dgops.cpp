@@ -394,6 +394,14 @@ namespace dgops
       require(c.time_factor == DG_TIME_FACTOR_ONE || c.final_time > 0,
               "final_time must be positive for the cos time factor");
       require(c.irk_stages >= 0 && c.irk_stages <= 5, "irk_stages 0..5");
+      require((c.use_gl_quad == 0 || c.use_gl_quad == 1) &&
+                (c.use_gl_quad_mass == 0 || c.use_gl_quad_mass == 1),
+              "use_gl_quad and use_gl_quad_mass are 0 or 1");
+      if (c.operator_kind == DG_OP_BURGERS)
+        require(!c.use_gl_quad && !c.use_gl_quad_mass,
+                "Burgers operator: QGauss quadratures only (burgers_solver.cc "
+                "has no quadrature switch)",
+                DG_ERR_UNSUPPORTED);
       if (c.velocity_kind == DG_VEL_CELLULAR_3D)
         require(c.dim == 3, "cellular velocity field is 3D");
       if (c.operator_kind == DG_OP_ADVECTION_DIFFUSION)
@@ -404,6 +412,33 @@ namespace dgops
                   DG_ERR_UNSUPPORTED);
     }
   } // namespace
+
+  // apply with the mass matrix integrated by QGaussLobatto(k+1) and the
+  // operator by QGauss(k+1) (use_gl_quad_mass alone): the operator's
+  // integration S^T (...) into dst, then the inverse of the Gauss-Lobatto
+  // mass matrix, diag(1 / JxW) at the nodes, in place
+  // (advection_solver.cc:661-691 with quadrature_mass of :443-445)
+  void
+  apply_gl_mass(Operator &op, const OpParams &P, const void *src, void *dst)
+  {
+    if (op.fp32())
+      {
+        OpBuffers<float> io = {};
+        io.src   = (const float *)src;
+        io.dst   = (float *)dst;
+        io.scale = 1.f;
+        launch_operator(op, P, 1, true, true, fin_integrate, epi_store, &io);
+      }
+    else
+      {
+        OpBuffers<double> io = {};
+        io.src   = (const double *)src;
+        io.dst   = (double *)dst;
+        io.scale = 1.;
+        launch_operator(op, P, 1, false, true, fin_integrate, epi_store, &io);
+      }
+    aux_inverse_mass(op, op.cfg.precision, dst);
+  }
 } // namespace dgops
 
 using namespace dgops;
@@ -464,7 +499,8 @@ dg_create(const dg_config *cfg, void *stream, dg_operator **out)
     op->n           = cfg->degree + 1;
     op->nd          = cfg->dim == 3 ? op->n * op->n * op->n : op->n * op->n;
     op->n_dofs      = op->mesh.n_cells * op->nd;
-    op->basis       = make_basis(op->n);
+    op->basis       = make_basis(op->n, cfg->use_gl_quad != 0);
+    op->gl_mass_only = !cfg->use_gl_quad && cfg->use_gl_quad_mass;
     if (cfg->operator_kind == DG_OP_BURGERS)
       {
         // QGauss(k(k+1)/2) tables (burgers_solver.cc:234)
@@ -512,6 +548,20 @@ dg_create(const dg_config *cfg, void *stream, dg_operator **out)
     hip_check(hipMemcpy(op->d_basis_f.ptr, &bf, sizeof(bf),
                         hipMemcpyHostToDevice),
               "upload basis");
+    if (op->gl_mass_only)
+      {
+        const Basis1D mb  = make_basis(op->n, true);
+        const auto    mbd = make_dev_basis<double>(mb);
+        const auto    mbf = make_dev_basis<float>(mb);
+        op->d_mbasis_d.alloc(sizeof(mbd));
+        op->d_mbasis_f.alloc(sizeof(mbf));
+        hip_check(hipMemcpy(op->d_mbasis_d.ptr, &mbd, sizeof(mbd),
+                            hipMemcpyHostToDevice),
+                  "upload mass basis");
+        hip_check(hipMemcpy(op->d_mbasis_f.ptr, &mbf, sizeof(mbf),
+                            hipMemcpyHostToDevice),
+                  "upload mass basis");
+      }
     const size_t nb = op->mesh.neighbors.size() * sizeof(int);
     const size_t nv = op->mesh.cell_vertices.size() * sizeof(double);
     op->d_neighbors.alloc(nb);
@@ -582,6 +632,11 @@ dg_apply(dg_operator *h, const void *src, void *dst, double t)
             "factor_skew = 0",
             DG_ERR_UNSUPPORTED);
     OpParams P = op->base_params(t, 1);
+    if (op->gl_mass_only)
+      {
+        apply_gl_mass(*op, P, src, dst);
+        return;
+      }
     if (op->fp32())
       {
         OpBuffers<float> io = {};
@@ -638,6 +693,16 @@ namespace dgops
             "perform_stage: the fused LSRK stage is built in FP64 (create "
             "the operator with precision DG_FP64)",
             DG_ERR_UNSUPPORTED);
+    if (op.gl_mass_only)
+      {
+        // two passes: k = M_GL^-1 (-L ri_in), then the stage update
+        const size_t bytes = sizeof(double) * op.n_dofs;
+        op.d_gl_k.alloc(bytes);
+        apply_gl_mass(op, P, ri_in, op.d_gl_k.ptr);
+        vec_lsrk_update(op, DG_FP64, op.n_dofs, b_dt, a_dt, op.d_gl_k.ptr,
+                        sol_in, sol_out, ri_out);
+        return;
+      }
     {
         OpBuffers<double> io = {};
         io.src     = (const double *)ri_in;

@@ -102,9 +102,22 @@ namespace dgops
         }
     }
 
+    // the quadrature switches (use_gl_quad, use_gl_quad_mass) are built for
+    // the explicit operators; the implicit operators, their FDM and the IRK
+    // path integrate with QGauss(k+1)
+    void
+    require_gauss_quadrature(const Operator &op)
+    {
+      require(!op.cfg.use_gl_quad && !op.cfg.use_gl_quad_mass,
+              "implicit operators: built for QGauss(k+1) (use_gl_quad and "
+              "use_gl_quad_mass apply to the explicit operator)",
+              DG_ERR_UNSUPPORTED);
+    }
+
     void
     setup_irk(Operator &op)
     {
+      require_gauss_quadrature(op);
       const int s = op.cfg.irk_stages;
       require(s >= 1 && s <= 4, "IRK needs 1..4 stages", DG_ERR_UNSUPPORTED);
       if (op.irk.s == s)
@@ -249,6 +262,7 @@ namespace dgops
     void
     setup_fdm(Operator &op)
     {
+      require_gauss_quadrature(op);
       if (op.d_fdm_d.ptr)
         return;
       const int      n     = op.n;
@@ -378,6 +392,7 @@ namespace dgops
       require(op.cfg.operator_kind != DG_OP_BURGERS,
               "implicit operator: advection (+ diffusion) only",
               DG_ERR_STATE);
+      require_gauss_quadrature(op);
       OpParams P = op.base_params(t + dt, 1);
       P.sign     = 1.;
       P.has_mass = (with_mass && dt != 0.) ? 1 : 0;

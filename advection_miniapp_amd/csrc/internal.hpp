@@ -109,6 +109,11 @@ namespace dgops
 
     Basis1D basis;
     DevBuf  d_basis_d, d_basis_f, d_neighbors, d_vertices;
+    // use_gl_quad_mass without use_gl_quad: the mass matrix is integrated
+    // with QGaussLobatto(k+1) (its basis in d_mbasis_*), the operator with
+    // QGauss(k+1); the apply then runs in two passes through d_gl_k
+    bool   gl_mass_only = false;
+    DevBuf d_mbasis_d, d_mbasis_f, d_gl_k;
 
     // implicit state
     double time      = 0;
@@ -145,6 +150,13 @@ namespace dgops
     const void *basis_ptr(bool fp32) const
     {
       return fp32 ? d_basis_f.ptr : d_basis_d.ptr;
+    }
+    // basis of the mass-matrix quadrature (inverse mass)
+    const void *mass_basis_ptr(bool fp32) const
+    {
+      if (!gl_mass_only)
+        return basis_ptr(fp32);
+      return fp32 ? d_mbasis_f.ptr : d_mbasis_d.ptr;
     }
     bool fp32() const { return cfg.precision == DG_FP32; }
     size_t scalar_size() const { return fp32() ? 4 : 8; }

@@ -238,9 +238,46 @@ def serving_requests(n_req, final_time, seed=2024, rate=200.0):
     return arrivals, reqs
 
 
+def serving_cpu_baseline(reqs, target_seconds=10.0):
+    """The oracle (restated reference algorithm, OpenMP over cells) on the
+    serving stream, on this box's host cores (at most the 16 of one GPU):
+    in arrival order, each request is set up and advanced by one LSRK(5,4)
+    step until about `target_seconds` of stepping time; the rate counts the
+    DoF of the 5 stage evaluations per step, as the GPU figures do."""
+    sys.path.insert(0, ROOT)
+    import oracle
+    from advection_miniapp_amd.serving import request_config
+    from tests.helpers import Cfg
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    oracle.load().or_set_threads(cores)
+    work, el, used = 0.0, 0.0, 0
+    for r in reqs:
+        cfg, dt, _ = request_config(r)
+        c = Cfg()
+        for name, _ in oracle.OrProblem._fields_:
+            v = getattr(cfg, name)
+            setattr(c, name, list(v) if not isinstance(v, (int, float)) else v)
+        o = c.oracle_handle()
+        u = o.project(0.0)
+        r1, r2 = o.zeros(), o.zeros()
+        t0 = time.perf_counter()
+        o.lsrk_time_step(1, 0.0, dt, u, r1, r2)
+        el += time.perf_counter() - t0
+        work += 5.0 * o.n_dofs
+        used += 1
+        del o, u, r1, r2
+        if el > target_seconds:
+            break
+    return dict(value=work / el, unit="DoF/s", cores=cores, kind="port",
+                sample=f"one LSRK(5,4) step of each of the first {used} "
+                       f"requests in arrival order ({el:.1f} s of "
+                       "stepping; set-up not timed)")
+
+
 def time_serving(args):
     """Continuous batching (Server) against serial one-at-a-time solves of
-    the same arrival stream; latency = completion - arrival."""
+    the same arrival stream; latency = completion - arrival. With the CPU
+    leg on, the oracle's rate on the same stream is timed beside them."""
     import numpy as np
     import torch
     from advection_miniapp_amd import (AdvectionOperation,
@@ -299,7 +336,11 @@ def time_serving(args):
     lat_s = np.array(lat_s)
     span_s = finish - arrivals[0]
     tot = float(sum(work))
+    extra = {}
+    if not args.no_cpu:
+        extra["cpu_baseline"] = serving_cpu_baseline(reqs)
     return dict(
+        **extra,
         workload=f"{len(reqs)} requests, Poisson 200/s (seed 2024), degree "
                  "2..6, 8..48 cells per direction, T = "
                  f"{args.serve_final_time}, LSRK(5,4) at Courant 0.3",

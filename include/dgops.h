@@ -120,6 +120,13 @@ typedef struct dg_config
   /* Radau IIA stages of the IRK entry points (1..4; dg_irk_coefficients
      alone also tabulates 5)                                               */
   int      irk_stages;
+  /* quadrature switches of advection_solver.cc:110-116 (:440-445):
+     use_gl_quad integrates the operator and the mass matrix with
+     QGaussLobatto(k+1) instead of QGauss(k+1); use_gl_quad_mass only the
+     mass matrix (inverse mass, and the apply and stages through it).
+     Not for the Burgers operator.                                         */
+  int      use_gl_quad;
+  int      use_gl_quad_mass;
 } dg_config;
 
 typedef struct dg_info

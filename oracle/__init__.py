@@ -39,6 +39,8 @@ class OrProblem(ctypes.Structure):
         ("solution_kind", ctypes.c_int),
         ("solution_param", ctypes.c_double * 8),
         ("irk_stages", ctypes.c_int),
+        ("use_gl_quad", ctypes.c_int),
+        ("use_gl_quad_mass", ctypes.c_int),
     ]
 
 

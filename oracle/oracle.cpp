@@ -281,8 +281,11 @@ struct or_handle
   std::vector<int>    ijk, nbr;
   std::vector<double> corners; // [cell][2^dim][dim]
   double              min_dist;
-  // 1D data
+  // 1D data: the operator's quadrature (QGauss(n), or QGaussLobatto(n)
+  // under use_gl_quad) and the mass matrix's (xqm, wqm, Sm, Sminv: the
+  // operator's, or QGaussLobatto(n) under use_gl_quad_mass)
   std::vector<double> xq, wq, S, G, Sinv, phi0, phi1;
+  std::vector<double> xqm, wqm, Sm, Sminv;
   std::vector<ld>     xgl;
 
   double time_factor(double t) const
@@ -1151,20 +1154,23 @@ namespace
 #pragma omp for schedule(static)
       for (int64_t cell = 0; cell < H.n_cells; ++cell)
         {
-          // S^-T: (S^-1)^T applied as the transpose of Sinv
-          const double *Mi[3] = {H.Sinv.data(), H.Sinv.data(), H.Sinv.data()};
+          // S^-T: (S^-1)^T applied as the transpose of Sinv, with S the
+          // nodal basis at the mass matrix's quadrature points
+          const double *Mi[3] = {H.Sminv.data(), H.Sminv.data(),
+                                 H.Sminv.data()};
           tensor_apply(dim, Mi, n, n, true, src + cell * nd, a.data(), t1, t2);
           for (int q = 0; q < nd; ++q)
             {
               int qi[3];
               q_index(H, q, qi);
-              double xi[3] = {H.xq[qi[0]], H.xq[qi[1]], dim == 3 ? H.xq[qi[2]] : 0};
+              double xi[3] = {H.xqm[qi[0]], H.xqm[qi[1]],
+                              dim == 3 ? H.xqm[qi[2]] : 0};
               double x[3], J[3][3], Ji[3][3];
               map_point(H, cell, xi, x, J);
               const double det = inverse_jacobian(dim, J, Ji);
-              double       w   = H.wq[qi[0]] * H.wq[qi[1]];
+              double       w   = H.wqm[qi[0]] * H.wqm[qi[1]];
               if (dim == 3)
-                w *= H.wq[qi[2]];
+                w *= H.wqm[qi[2]];
               a[q] /= det * w;
             }
           tensor_apply(dim, Mi, n, n, false, a.data(), b.data(), t1, t2);
@@ -1329,6 +1335,19 @@ or_create(const or_problem *p)
   std::vector<ld> x, w;
   gauss(n, x, w);
   H->xgl = lobatto(n);
+  // Gauss-Lobatto weights: the integrals of the nodal Lagrange polynomials
+  // (degree n-1), by the n-point Gauss rule
+  std::vector<ld> wgl(n, 0);
+  for (int i = 0; i < n; ++i)
+    for (int q = 0; q < n; ++q)
+      wgl[i] += w[q] * lagr(H->xgl, i, x[q]);
+  const bool gl_op = p->use_gl_quad != 0;
+  const bool gl_m  = gl_op || p->use_gl_quad_mass != 0;
+  if (gl_op)
+    {
+      x = H->xgl;
+      w = wgl;
+    }
   H->xq.resize(n);
   H->wq.resize(n);
   for (int i = 0; i < n; ++i)
@@ -1352,6 +1371,27 @@ or_create(const or_problem *p)
       H->phi1[i] = (double)lagr(H->xgl, i, 1);
     }
   H->Sinv = invert(H->S, n);
+  if (gl_m && !gl_op)
+    {
+      H->xqm.resize(n);
+      H->wqm.resize(n);
+      H->Sm.resize(n * n);
+      for (int q = 0; q < n; ++q)
+        {
+          H->xqm[q] = (double)H->xgl[q];
+          H->wqm[q] = (double)wgl[q];
+          for (int i = 0; i < n; ++i)
+            H->Sm[q * n + i] = (double)lagr(H->xgl, i, H->xgl[q]);
+        }
+      H->Sminv = invert(H->Sm, n);
+    }
+  else
+    {
+      H->xqm   = H->xq;
+      H->wqm   = H->wq;
+      H->Sm    = H->S;
+      H->Sminv = H->Sinv;
+    }
   return H;
 }
 
@@ -1415,27 +1455,28 @@ or_inverse_mass(or_handle *h, const double *src, double *dst)
 void
 or_mass(or_handle *h, const double *src, double *dst)
 {
-  const int n = h->n, nd = h->nd, dim = h->dim;
+  // the mass matrix of the mass quadrature: S^T diag(JxW) S
+  const int           n = h->n, nd = h->nd, dim = h->dim;
+  std::vector<double> a(nd), t1, t2;
+  const double       *Ms[3] = {h->Sm.data(), h->Sm.data(), h->Sm.data()};
   for (int64_t cell = 0; cell < h->n_cells; ++cell)
     {
-      CellEval E;
-      evaluate(*h, src + cell * nd, E, false);
-      std::vector<double> v(nd);
+      tensor_apply(dim, Ms, n, n, false, src + cell * nd, a.data(), t1, t2);
       for (int q = 0; q < nd; ++q)
         {
           int qi[3];
           q_index(*h, q, qi);
-          double xi[3] = {h->xq[qi[0]], h->xq[qi[1]], dim == 3 ? h->xq[qi[2]] : 0};
+          double xi[3] = {h->xqm[qi[0]], h->xqm[qi[1]],
+                          dim == 3 ? h->xqm[qi[2]] : 0};
           double x[3], J[3][3], Ji[3][3];
           map_point(*h, cell, xi, x, J);
           const double det = inverse_jacobian(dim, J, Ji);
-          double       w   = h->wq[qi[0]] * h->wq[qi[1]] * (dim == 3 ? h->wq[qi[2]] : 1);
-          v[q]             = E.vals[q] * det * w;
+          double       w   = h->wqm[qi[0]] * h->wqm[qi[1]] *
+                     (dim == 3 ? h->wqm[qi[2]] : 1);
+          a[q] *= det * w;
         }
-      std::fill(dst + cell * nd, dst + (cell + 1) * nd, 0.);
-      integrate_add(*h, v.data(), nullptr, dst + cell * nd, E);
+      tensor_apply(dim, Ms, n, n, true, a.data(), dst + cell * nd, t1, t2);
     }
-  (void)n;
 }
 
 void

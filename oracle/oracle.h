@@ -32,6 +32,9 @@ typedef struct or_problem
   int      solution_kind;
   double   solution_param[8];
   int      irk_stages;
+  /* QGaussLobatto(k+1) for the operator and the mass matrix, or for the
+     mass matrix only (advection_solver.cc:110-116, :440-445) */
+  int      use_gl_quad, use_gl_quad_mass;
 } or_problem;
 
 typedef struct or_handle or_handle;

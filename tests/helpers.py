@@ -9,7 +9,8 @@ DEFAULTS = dict(
     periodic=[1, 1, 1], perturbation=0.0, seed=42, velocity_kind=0,
     velocity=[1.0, 0.0, 0.0], time_factor=0, final_time=1.0, flux_alpha=1.0,
     factor_skew=0.0, viscosity=0.0, penalty_factor=0.0, solution_kind=0,
-    solution_param=[0.0] * 8, irk_stages=3)
+    solution_param=[0.0] * 8, irk_stages=3, use_gl_quad=0,
+    use_gl_quad_mass=0)
 
 
 class Cfg:

@@ -115,6 +115,91 @@ def test_known_answer_polynomial(dim, k):
     assert rel_diff(u, (s ** k).ravel()) < 1e-12
 
 
+@pytest.mark.parametrize("dim,k", [(2, 1), (2, 3), (3, 2), (3, 4)])
+def test_gl_quadrature_known_answer_polynomial(dim, k):
+    """use_gl_quad (advection_solver.cc:110-112): with QGaussLobatto(k+1)
+    the quadrature points are the nodes. On axis-aligned cells every term
+    of the x part reduces to w_b w_c times a line integral along x whose
+    integrand (phi' u or phi u', degree 2k-1) the rule integrates exactly,
+    and the lumped mass matrix is diag(w_a w_b w_c |K|): M^-1(-L u) =
+    -a.grad u at the nodes for u in Q_k (summation by parts), although the
+    mass matrix itself is inexact."""
+    c = _poly_problem(dim, k, 0.0)
+    c.use_gl_quad = 1
+    o = c.oracle_handle()
+    u = o.project(0.0)
+    du = o.apply(u, 0.0)
+    pts = nodal_points(o.cell_vertices(), k + 1, dim)
+    a = np.array(c.velocity[:dim])
+    s = 1 + sum((d + 1) * pts[..., d] for d in range(dim))
+    grad_dot = k * s ** (k - 1) * sum((d + 1) * a[d] for d in range(dim))
+    assert rel_diff(du, -grad_dot.ravel()) < 1e-11
+    assert rel_diff(u, (s ** k).ravel()) < 1e-12
+    # the Gauss-Lobatto mass matrix is not the exact one (degree 2k)
+    g = _poly_problem(dim, k, 0.0).oracle_handle()
+    assert rel_diff(o.mass(u), g.mass(u)) > 1e-6
+
+
+def _gl_weights(n):
+    """Gauss-Lobatto weights on [0,1] in closed form, 1 / (n (n-1)
+    P_{n-1}(x_i)^2), at the roots of P'_{n-1} and the end points."""
+    from numpy.polynomial import legendre as leg
+    c = np.zeros(n)
+    c[-1] = 1
+    x = np.concatenate([[-1.0], np.sort(leg.legroots(leg.legder(c))), [1.0]])
+    w = 1.0 / (n * (n - 1) * leg.legval(x, c) ** 2)
+    return (x + 1) / 2, w
+
+
+@pytest.mark.parametrize("dim,k,pert", [(2, 3, 0.0), (3, 4, 0.0),
+                                        (3, 3, 0.15)])
+def test_gl_mass_inverse_closed_form(dim, k, pert):
+    """use_gl_quad_mass (advection_solver.cc:114-116): the mass matrix of
+    QGaussLobatto(k+1) is diagonal in the nodal basis, so its inverse is
+    v / (w_a w_b (w_c) det J) with det J at the nodes, from the closed-form
+    weights and the multilinear map of the exported corners."""
+    n = k + 1
+    c = Cfg(dim=dim, degree=k, refine=1, perturbation=pert, seed=5,
+            use_gl_quad_mass=1, lo=[-0.5, 0, 0.25], hi=[0.5, 1.5, 1.0])
+    o = c.oracle_handle()
+    v = np.random.default_rng(2).standard_normal(o.n_dofs)
+    x, w = _gl_weights(n)
+    verts = o.cell_vertices().reshape(o.n_cells, 2 ** dim, dim)
+    jxw = np.empty((o.n_cells, n ** dim))
+    for p in range(n ** dim):
+        idx = [(p // n ** d) % n for d in range(dim)]
+        xi = [x[i] for i in idx]
+        # multilinear map: J[e][d] = d x_e / d xi_d
+        J = np.zeros((o.n_cells, dim, dim))
+        for vtx in range(2 ** dim):
+            bits = [(vtx >> d) & 1 for d in range(dim)]
+            for d in range(dim):
+                dphi = 1.0
+                for e in range(dim):
+                    if e == d:
+                        dphi *= 1.0 if bits[e] else -1.0
+                    else:
+                        dphi *= xi[e] if bits[e] else 1 - xi[e]
+                J[:, :, d] += verts[:, vtx, :] * dphi
+        jxw[:, p] = np.linalg.det(J) * np.prod([w[i] for i in idx])
+    assert rel_diff(o.inverse_mass(v), (v.reshape(jxw.shape) / jxw).ravel()) \
+        < 1e-13
+    assert rel_diff(o.mass(o.inverse_mass(v)), v) < 1e-13
+
+
+def test_gl_mass_apply_is_gauss_apply_remassed():
+    """With use_gl_quad_mass the operator keeps QGauss(k+1) and only the
+    inverse mass changes: apply_GL(u) = M_GL^-1 M_Gauss apply_Gauss(u),
+    M_Gauss from the plain problem, M_GL^-1 checked in closed form above."""
+    kw = dict(dim=3, degree=3, refine=1, perturbation=0.1, velocity_kind=1)
+    g = Cfg(**kw).oracle_handle()
+    m = Cfg(use_gl_quad_mass=1, **kw).oracle_handle()
+    u = np.random.default_rng(4).standard_normal(g.n_dofs)
+    ref = m.inverse_mass(g.mass(g.apply(u, 0.1)))
+    assert rel_diff(m.apply(u, 0.1), ref) < 1e-12
+    assert rel_diff(m.apply(u, 0.1), g.apply(u, 0.1)) > 1e-4
+
+
 def _linear_problem(dim, k, pert):
     """Globally linear u = 1 + x + 2y + 3z on perturbed (non-affine) cells
     of a box, constant speed, Dirichlet data = u."""
