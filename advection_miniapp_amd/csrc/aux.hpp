@@ -20,6 +20,10 @@ namespace dgops
   // vec = M^-1 vec in place, precision p
   // (CellwiseInverseMassMatrix::apply, advection_solver_implicit_real.cc:894-909)
   void aux_inverse_mass(const Operator &op, int p, void *vec);
+  // dst += scale * (SIPG diffusion terms of src, integrated form, sign
+  // P.sign) on cells of any shape (FP64)
+  void aux_sipg_general(const Operator &op, const OpParams &P,
+                        const void *src, void *dst, double scale);
 
   // VTU file of a host solution vector (vtu.cpp)
   void write_vtu_host(const dg_config &cfg, const StructuredMesh &mesh,

@@ -294,13 +294,15 @@ namespace dgops
                   bool            skew,
                   int             fin,
                   int             epi,
-                  const void     *io)
+                  const void     *io,
+                  bool            advection_only)
   {
-    const int diff = op.cfg.operator_kind == DG_OP_ADVECTION_DIFFUSION;
+    const int diff =
+      op.cfg.operator_kind == DG_OP_ADVECTION_DIFFUSION && !advection_only;
     if (diff && !P.cartesian)
       throw Error(DG_ERR_UNSUPPORTED,
-                  "advection-diffusion (SIPG) is built for axis-aligned "
-                  "meshes (perturbation 0)");
+                  "the fused SIPG operator variants are built for "
+                  "axis-aligned meshes (perturbation 0)");
     // axis-aligned cells with a constant transport speed need no tables
     // (products of 1D weights); SIPG with a variable speed evaluates it per
     // point; everything else reads the precomputed geometry tables
@@ -413,16 +415,29 @@ namespace dgops
     }
   } // namespace
 
-  // apply with the mass matrix integrated by QGaussLobatto(k+1) and the
-  // operator by QGauss(k+1) (use_gl_quad_mass alone): the operator's
-  // integration S^T (...) into dst, then the inverse of the Gauss-Lobatto
-  // mass matrix, diag(1 / JxW) at the nodes, in place
-  // (advection_solver.cc:661-691 with quadrature_mass of :443-445)
-  void
-  apply_gl_mass(Operator &op, const OpParams &P, const void *src, void *dst)
+  // the apply in separate passes (advection_solver.cc:661-691: the loop,
+  // then the inverse mass): the operator's integration S^T (...) into dst,
+  // the SIPG diffusion terms on perturbed cells added to it, then the
+  // inverse mass matrix in place. Used when the mass matrix is integrated
+  // by QGaussLobatto(k+1) and the operator by QGauss(k+1) (use_gl_quad_mass
+  // alone; the inverse mass is then diag(1 / JxW) at the nodes, :443-445),
+  // and for SIPG on perturbed cells.
+  bool
+  two_pass_apply(const Operator &op)
   {
+    return op.gl_mass_only || op.sipg_general();
+  }
+
+  void
+  apply_two_pass(Operator &op, const OpParams &P, const void *src, void *dst)
+  {
+    const bool general = op.sipg_general();
     if (op.fp32())
       {
+        require(!general,
+                "SIPG on perturbed cells is built in FP64 (the FP32 "
+                "variants need axis-aligned cells)",
+                DG_ERR_UNSUPPORTED);
         OpBuffers<float> io = {};
         io.src   = (const float *)src;
         io.dst   = (float *)dst;
@@ -435,7 +450,10 @@ namespace dgops
         io.src   = (const double *)src;
         io.dst   = (double *)dst;
         io.scale = 1.;
-        launch_operator(op, P, 1, false, true, fin_integrate, epi_store, &io);
+        launch_operator(op, P, 1, false, true, fin_integrate, epi_store, &io,
+                        general);
+        if (general)
+          aux_sipg_general(op, P, src, dst, 1.);
       }
     aux_inverse_mass(op, op.cfg.precision, dst);
   }
@@ -632,9 +650,9 @@ dg_apply(dg_operator *h, const void *src, void *dst, double t)
             "factor_skew = 0",
             DG_ERR_UNSUPPORTED);
     OpParams P = op->base_params(t, 1);
-    if (op->gl_mass_only)
+    if (two_pass_apply(*op))
       {
-        apply_gl_mass(*op, P, src, dst);
+        apply_two_pass(*op, P, src, dst);
         return;
       }
     if (op->fp32())
@@ -693,12 +711,12 @@ namespace dgops
             "perform_stage: the fused LSRK stage is built in FP64 (create "
             "the operator with precision DG_FP64)",
             DG_ERR_UNSUPPORTED);
-    if (op.gl_mass_only)
+    if (two_pass_apply(op))
       {
-        // two passes: k = M_GL^-1 (-L ri_in), then the stage update
+        // two passes: k = M^-1 (-L ri_in), then the stage update
         const size_t bytes = sizeof(double) * op.n_dofs;
         op.d_gl_k.alloc(bytes);
-        apply_gl_mass(op, P, ri_in, op.d_gl_k.ptr);
+        apply_two_pass(op, P, ri_in, op.d_gl_k.ptr);
         vec_lsrk_update(op, DG_FP64, op.n_dofs, b_dt, a_dt, op.d_gl_k.ptr,
                         sol_in, sol_out, ri_out);
         return;

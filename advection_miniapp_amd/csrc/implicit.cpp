@@ -397,8 +397,15 @@ namespace dgops
       P.sign     = 1.;
       P.has_mass = (with_mass && dt != 0.) ? 1 : 0;
       P.mass[0][0] = dt != 0. ? 1. / dt : 0.;
+      // SIPG on perturbed cells: the advection variant, then the
+      // general-geometry diffusion terms added (FP64)
+      const bool general = op.sipg_general();
       if (fp32)
         {
+          require(!general,
+                  "SIPG on perturbed cells is built in FP64 (the FP32 "
+                  "variants need axis-aligned cells)",
+                  DG_ERR_UNSUPPORTED);
           OpBuffers<float> io = {};
           io.src   = (const float *)src;
           io.dst   = (float *)dst;
@@ -412,7 +419,9 @@ namespace dgops
           io.dst   = (double *)dst;
           io.scale = scale;
           launch_operator(op, P, 1, false, true, fin_integrate, epi_store,
-                          &io);
+                          &io, general);
+          if (general)
+            aux_sipg_general(op, P, src, dst, scale);
         }
     }
 
@@ -422,6 +431,9 @@ namespace dgops
               void *dst)
     {
       setup_irk(op);
+      require(!op.sipg_general(),
+              "IRK operators with SIPG need axis-aligned cells",
+              DG_ERR_UNSUPPORTED);
       const int s = op.irk.s;
       OpParams  P = op.base_params(t, s);
       P.sign      = 1.;
@@ -468,6 +480,9 @@ namespace dgops
                   const void *src, void *dst)
     {
       setup_irk(op);
+      require(!op.sipg_general(),
+              "IRK operators with SIPG need axis-aligned cells",
+              DG_ERR_UNSUPPORTED);
       const int s = op.irk.s;
       OpParams  P = op.base_params(t, s);
       P.sign      = 1.;

@@ -111,7 +111,8 @@ namespace dgops
     DevBuf  d_basis_d, d_basis_f, d_neighbors, d_vertices;
     // use_gl_quad_mass without use_gl_quad: the mass matrix is integrated
     // with QGaussLobatto(k+1) (its basis in d_mbasis_*), the operator with
-    // QGauss(k+1); the apply then runs in two passes through d_gl_k
+    // QGauss(k+1). The apply then runs in two passes; stages keep their
+    // right-hand side in d_gl_k (also for SIPG on perturbed cells)
     bool   gl_mass_only = false;
     DevBuf d_mbasis_d, d_mbasis_f, d_gl_k;
 
@@ -160,10 +161,20 @@ namespace dgops
     }
     bool fp32() const { return cfg.precision == DG_FP32; }
     size_t scalar_size() const { return fp32() ? 4 : 8; }
+    // SIPG on perturbed (non-axis-aligned) cells: the advection operator
+    // kernel (without its diffusion terms) plus the general-geometry SIPG
+    // kernel, then the inverse mass
+    bool sipg_general() const
+    {
+      return cfg.operator_kind == DG_OP_ADVECTION_DIFFUSION &&
+             cfg.perturbation != 0.;
+    }
     void  *solver_vector(size_t idx, size_t bytes);
   };
 
   // kernels launched from host code
+  // advection_only: the advection variant also for an advection-diffusion
+  // operator (its diffusion terms come from aux_sipg_general)
   void launch_operator(const Operator &op,
                        const OpParams &P,
                        int             nc,
@@ -171,7 +182,8 @@ namespace dgops
                        bool            skew,
                        int             fin,
                        int             epi,
-                       const void     *io);
+                       const void     *io,
+                       bool            advection_only = false);
 
   // the tile shape for a launch of variant `base_key` (shape bits 0):
   // forced, default, or tuned on first use per device and mesh size class

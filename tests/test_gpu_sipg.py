@@ -95,3 +95,96 @@ def test_sipg_implicit_vmult_matches_oracle(gpu_lib):
     dst = op.initialize_dof_vector()
     op.vmult(dst, dev(u))
     assert rel_diff(host(dst), o.vmult(u, 0.2, 0.05)) < 1e-12
+
+
+# SIPG on perturbed (non-axis-aligned) cells: advection operator kernel in
+# integrate form + the general-geometry diffusion kernel + inverse mass
+PERTURBED = {
+    "3d_k3_const": dict(dim=3, k=3, perturbation=0.1, viscosity=0.05),
+    "3d_k4_cellular": dict(dim=3, k=4, perturbation=0.15, viscosity=0.02,
+                           velocity_kind=1),
+    "3d_k2_pure_diffusion": dict(dim=3, k=2, perturbation=0.1,
+                                 viscosity=1.0, velocity=[0, 0, 0]),
+    "3d_k7_const": dict(dim=3, k=7, perturbation=0.1, viscosity=0.01,
+                        refine=1),
+    "2d_k4_vortex": dict(dim=2, k=4, perturbation=0.1, viscosity=0.05,
+                         velocity_kind=2, time_factor=1, final_time=8.0),
+    "2d_k1": dict(dim=2, k=1, perturbation=0.2, viscosity=0.1),
+}
+
+
+def sipg_perturbed(name):
+    kw = dict(PERTURBED[name])
+    dim, k = kw.pop("dim"), kw.pop("k")
+    kw.setdefault("seed", 7)
+    return sipg(dim, k, **kw)
+
+
+@pytest.mark.parametrize("name", sorted(PERTURBED))
+def test_sipg_perturbed_apply_matches_oracle(gpu_lib, name):
+    c = sipg_perturbed(name)
+    print(f"case {name}", flush=True)
+    op, o = AdvectionOperation(c.dg()), c.oracle_handle()
+    u = np.random.default_rng(15).standard_normal(o.n_dofs)
+    for t in (0.0, 0.3):
+        dst = op.initialize_dof_vector()
+        op.apply(dev(u), dst, t)
+        assert rel_diff(host(dst), o.apply(u, t)) < 1e-12, (name, t)
+
+
+@pytest.mark.parametrize("name", ["3d_k3_const", "2d_k4_vortex"])
+def test_sipg_perturbed_stage_and_vmult_match_oracle(gpu_lib, name):
+    c = sipg_perturbed(name)
+    op, o = AdvectionOperation(c.dg()), c.oracle_handle()
+    rng = np.random.default_rng(16)
+    ri, sol = rng.standard_normal((2, o.n_dofs))
+    ki_o, sol_o, nri_o = np.zeros(o.n_dofs), sol.copy(), np.zeros(o.n_dofs)
+    o.perform_stage(0.1, 0.01, 0.02, ri, ki_o, sol_o, nri_o)
+    d_sol, d_ki, d_nri = dev(sol), op.initialize_dof_vector(), \
+        op.initialize_dof_vector()
+    op.perform_stage(0.1, 0.01, 0.02, dev(ri), d_ki, d_sol, d_nri)
+    assert rel_diff(host(d_sol), sol_o) < 1e-12
+    assert rel_diff(host(d_nri), nri_o) < 1e-12
+    # implicit operator M/dt + L(t + dt)
+    op.set_time(0.2, 0.05)
+    dst = op.initialize_dof_vector()
+    op.vmult(dst, dev(ri))
+    assert rel_diff(host(dst), o.vmult(ri, 0.2, 0.05)) < 1e-12
+
+
+def test_sipg_perturbed_mesh_converges(gpu_lib):
+    """Pure diffusion (nu = 1) at degree 4 on cells perturbed by 10% of h:
+    the HIP M^-1(-L u) of u = prod sin(2 pi x_d) approaches nu lap u at
+    about order k - 1 (the oracle's pin of the same case)."""
+    from helpers import nodal_points
+    k, errs = 4, []
+    for r in (1, 2, 3):
+        c = Cfg(dim=3, degree=k, refine=r, perturbation=0.1, seed=7,
+                velocity=[0, 0, 0], operator_kind=1, viscosity=1.0,
+                penalty_factor=2 * k * (k + 1), solution_kind=2,
+                solution_param=[2 * np.pi, 0.0])
+        op = AdvectionOperation(c.dg())
+        u = op.initialize_dof_vector()
+        op.project_initial(u, 0.0)
+        du = op.initialize_dof_vector()
+        op.apply(u, du, 0.0)
+        p = nodal_points(op.cell_vertices(), k + 1, 3)
+        ex = (-3 * (2 * np.pi) ** 2 * np.prod(np.sin(2 * np.pi * p), -1))
+        ex = ex.ravel()
+        d = host(du)
+        errs.append(np.sqrt(np.mean((d - ex) ** 2) / np.mean(ex ** 2)))
+        del op, u, du
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(rates > k - 2), (errs, rates)
+    assert errs[-1] < 3e-3, errs
+
+
+def test_sipg_perturbed_fp32_and_irk_are_refused(gpu_lib):
+    from advection_miniapp_amd import _lib as L
+    c = sipg_perturbed("3d_k3_const")
+    c.precision = 1
+    op = AdvectionOperation(c.dg())
+    dst = op.initialize_dof_vector()
+    with pytest.raises(L.DgError) as e:
+        op.apply(op.initialize_dof_vector(), dst, 0.0)
+    assert e.value.code == L.DG_ERR_UNSUPPORTED
