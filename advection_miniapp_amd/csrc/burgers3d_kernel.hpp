@@ -164,111 +164,133 @@ namespace dgops
                   "face buffers overlay the cell-term buffers");
     double *FT  = W;
     double *FUV = W + 12 * n * m;
+    // per-thread tasks, fixed for every phase below: each phase has at most
+    // R tasks (R >= n m, m^2, n^2), one per thread, and a task covers all
+    // components and sides
+    constexpr int UM = m <= 12 ? m : 4; // unroll of the loops over m points
+    const bool    t_nm = r < n * m, t_m2 = r < m2, t_n2 = r < n2;
+    const int     xj = r / m, xq = r % m; // (j or b, qx or qa)
+    const int     yq = r % m, yy = r / m; // (qa, qb) or (qx, qy)
+    const int     gi = r % n, gq = r / n; // (i, qy)
+    const int     hi = r % n, hj = r / n; // (i, j) or (a, b)
     auto    face_pair = [&](auto dc) {
       constexpr int d = decltype(dc)::value;
       using FI        = FaceIdx<n, 2 * d>; // strides of direction d
       constexpr int ta = FI::ta, tb = FI::tb;
-      // along ta
-      for (int t = r; t < 12 * n * m; t += R)
+      // along ta: FT[(s 2 + side) 3 + c][b m + qa], thread (b, qa)
+      if (t_nm)
         {
-          const int k = t / (n * m); // (s 2 + side) 3 + c
-          const int b = (t / m) % n, qa = t % m;
-          const int s = k / 6, sd = (k / 3) % 2, c = k % 3;
-          double    v = 0;
-          if (sd == 0)
+          const int b = xj, qa = xq;
+#pragma unroll
+          for (int k = 0; k < 12; ++k)
             {
-              const int layer = s ? n - 1 : 0;
+              const int s = k / 6, sd = (k / 3) % 2, c = k % 3;
+              double    v = 0;
 #pragma unroll
               for (int a = 0; a < n; ++a)
-                v = fma(sS[qa * n + a], U[c * n3 + FI::node(a, b, layer)], v);
+                {
+                  const double x =
+                    sd == 0 ? U[c * n3 + FI::node(a, b, s ? n - 1 : 0)] :
+                              NB[((2 * d + s) * 3 + c) * n2 + b * n + a];
+                  v = fma(sS[qa * n + a], x, v);
+                }
+              FT[k * n * m + b * m + qa] = v;
             }
-          else
-#pragma unroll
-            for (int a = 0; a < n; ++a)
-              v = fma(sS[qa * n + a],
-                      NB[((2 * d + s) * 3 + c) * n2 + b * n + a], v);
-          FT[k * n * m + b * m + qa] = v;
         }
       __syncthreads();
-      // along tb
-      for (int t = r; t < 12 * m2; t += R)
+      // along tb: thread (qa, qb)
+      if (t_m2)
         {
-          const int k  = t / m2;
-          const int qb = (t / m) % m, qa = t % m;
-          double    v  = 0;
+          const int qa = yq, qb = yy;
 #pragma unroll
-          for (int b = 0; b < n; ++b)
-            v = fma(sS[qb * n + b], FT[k * n * m + b * m + qa], v);
-          FUV[k * m2 + qb * m + qa] = v;
+          for (int k = 0; k < 12; ++k)
+            {
+              double v = 0;
+#pragma unroll
+              for (int b = 0; b < n; ++b)
+                v = fma(sS[qb * n + b], FT[k * n * m + b * m + qa], v);
+              FUV[k * m2 + qb * m + qa] = v;
+            }
         }
       __syncthreads();
-      // flux at the face points; the own slots receive -F n dA
-      for (int t = r; t < 2 * m2; t += R)
+      // flux at the face point of thread (qa, qb), both sides; the own
+      // slots receive -F n dA
+      if (t_m2)
         {
-          const int    s  = t / m2, pt = t % m2;
-          const int    qa = pt % m, qb = pt / m;
+          const int    qa = yq, qb = yy, pt = r;
           const double w  = sw[qa] * sw[qb];
-          double       nrm[3] = {0, 0, 0};
-          if constexpr (cartesian)
+#pragma unroll
+          for (int s = 0; s < 2; ++s)
             {
-              double area = 1;
-              for (int e = 0; e < 3; ++e)
-                if (e != d)
-                  area *= P.h[e];
-              nrm[d] = (s ? 1. : -1.) * area * w;
+              double nrm[3] = {0, 0, 0};
+              if constexpr (cartesian)
+                {
+                  double area = 1;
+                  for (int e = 0; e < 3; ++e)
+                    if (e != d)
+                      area *= P.h[e];
+                  nrm[d] = (s ? 1. : -1.) * area * w;
+                }
+              else
+                {
+                  double xi[3];
+                  xi[d]  = s;
+                  xi[ta] = sx[qa];
+                  xi[tb] = sx[qb];
+                  double x[3], J[3][3], adj[3][3];
+                  multilinear_map<3>(X, xi, x, J);
+                  adjugate<3>(J, adj);
+                  for (int e = 0; e < 3; ++e)
+                    nrm[e] = (s ? 1. : -1.) * adj[d][e] * w;
+                }
+              const double len =
+                sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+              double *own = FUV + s * 6 * m2, *nbr = own + 3 * m2;
+              double  um[3], up[3], unm = 0, unp = 0;
+              for (int c = 0; c < 3; ++c)
+                {
+                  um[c] = own[c * m2 + pt];
+                  up[c] = nbr[c * m2 + pt];
+                  unm += um[c] * nrm[c] / len;
+                  unp += up[c] * nrm[c] / len;
+                }
+              const double lam = P.alpha * fmax(fabs(unm), fabs(unp));
+              for (int c = 0; c < 3; ++c)
+                own[c * m2 + pt] =
+                  -0.5 * (unm * um[c] + unp * up[c] + lam * (um[c] - up[c])) *
+                  len;
             }
-          else
-            {
-              double xi[3];
-              xi[d]  = s;
-              xi[ta] = sx[qa];
-              xi[tb] = sx[qb];
-              double x[3], J[3][3], adj[3][3];
-              multilinear_map<3>(X, xi, x, J);
-              adjugate<3>(J, adj);
-              for (int e = 0; e < 3; ++e)
-                nrm[e] = (s ? 1. : -1.) * adj[d][e] * w;
-            }
-          const double len =
-            sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
-          double *own = FUV + s * 6 * m2, *nbr = own + 3 * m2;
-          double  um[3], up[3], unm = 0, unp = 0;
-          for (int c = 0; c < 3; ++c)
-            {
-              um[c] = own[c * m2 + pt];
-              up[c] = nbr[c * m2 + pt];
-              unm += um[c] * nrm[c] / len;
-              unp += up[c] * nrm[c] / len;
-            }
-          const double lam = P.alpha * fmax(fabs(unm), fabs(unp));
-          for (int c = 0; c < 3; ++c)
-            own[c * m2 + pt] =
-              -0.5 * (unm * um[c] + unp * up[c] + lam * (um[c] - up[c])) * len;
         }
       __syncthreads();
-      // integrate: along tb (into FT[s 3 + c]), then along ta onto the own
-      // layer of side s
-      for (int t = r; t < 6 * n * m; t += R)
+      // integrate: along tb (into FT[s 3 + c]), thread (b, qa) ...
+      if (t_nm)
         {
-          const int k = t / (n * m); // s 3 + c
-          const int b = (t / m) % n, qa = t % m;
-          const double *fl = FUV + ((k / 3) * 6 + k % 3) * m2;
-          double        v  = 0;
-#pragma unroll 4
-          for (int qb = 0; qb < m; ++qb)
-            v = fma(sS[qb * n + b], fl[qb * m + qa], v);
-          FT[k * n * m + b * m + qa] = v;
+          const int b = xj, qa = xq;
+#pragma unroll
+          for (int k = 0; k < 6; ++k)
+            {
+              const double *fl = FUV + ((k / 3) * 6 + k % 3) * m2 + qa;
+              double        v  = 0;
+#pragma unroll UM
+              for (int qb = 0; qb < m; ++qb)
+                v = fma(sS[qb * n + b], fl[qb * m], v);
+              FT[k * n * m + b * m + qa] = v;
+            }
         }
       __syncthreads();
-      for (int t = r; t < 6 * n2; t += R)
+      // ... then along ta onto the own layer of side s, thread (a, b)
+      if (t_n2)
         {
-          const int k = t / n2; // s 3 + c
-          const int b = (t / n) % n, a = t % n;
-          double    v = 0;
-#pragma unroll 4
-          for (int qa = 0; qa < m; ++qa)
-            v = fma(sS[qa * n + a], FT[k * n * m + b * m + qa], v);
-          RF[(k % 3) * n3 + FI::node(a, b, k / 3 ? n - 1 : 0)] += v;
+          const int a = hi, b = hj;
+#pragma unroll
+          for (int k = 0; k < 6; ++k)
+            {
+              double v = 0;
+#pragma unroll UM
+              for (int qa = 0; qa < m; ++qa)
+                v = fma(sS[qa * n + a], FT[k * n * m + b * m + qa], v);
+              RF[(k % 3) * n3 + FI::node(a, b, k / 3 ? n - 1 : 0)] += v;
+            }
         }
       __syncthreads();
     };
@@ -313,36 +335,42 @@ namespace dgops
         for (int e = 0; e < 3; ++e)
           adiag[e] = vol / P.h[e];
       }
+    // the slab phases use the per-thread tasks of the face phases: x
+    // interpolation (j, qx), y interpolation (qx, qy), x integration
+    // (i, qy), y integration (i, j)
 #pragma unroll 1
     for (int qz = 0; qz < m; ++qz)
       {
         // along x: T[c][j m + qx]
-        for (int t = r; t < 3 * n * m; t += R)
+        if (t_nm)
           {
-            const int c = t / (n * m), j = (t / m) % n, qx = t % m;
-            double    v = 0;
+            const double *w = W + qz * n2 + xj * n;
 #pragma unroll
-            for (int i = 0; i < n; ++i)
-              v = fma(sS[qx * n + i], W[(c * m + qz) * n2 + j * n + i], v);
-            T[c * n * m + j * m + qx] = v;
+            for (int c = 0; c < 3; ++c)
+              {
+                double v = 0;
+#pragma unroll
+                for (int i = 0; i < n; ++i)
+                  v = fma(sS[xq * n + i], w[c * m * n2 + i], v);
+                T[c * n * m + xj * m + xq] = v;
+              }
           }
         __syncthreads();
         // along y, then the point-wise fluxes: UV = (u_0, u_1, u_2, v_0,
         // v_1, v_2) with v = w adj(J) u
-        for (int t = r; t < m2; t += R)
+        if (t_m2)
           {
-            const int qx = t % m, qy = t / m;
-            double    u[3];
+            double u[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c)
               {
                 double v = 0;
 #pragma unroll
                 for (int j = 0; j < n; ++j)
-                  v = fma(sS[qy * n + j], T[c * n * m + j * m + qx], v);
+                  v = fma(sS[yy * n + j], T[c * n * m + j * m + yq], v);
                 u[c] = v;
               }
-            const double w = sw[qx] * sw[qy] * sw[qz];
+            const double w = sw[yq] * sw[yy] * sw[qz];
             double       v[3];
             if constexpr (cartesian)
               {
@@ -351,7 +379,7 @@ namespace dgops
               }
             else
               {
-                const double xi[3] = {sx[qx], sx[qy], sx[qz]};
+                const double xi[3] = {sx[yq], sx[yy], sx[qz]};
                 double       x[3], J[3][3], adj[3][3];
                 multilinear_map<3>(X, xi, x, J);
                 adjugate<3>(J, adj);
@@ -361,29 +389,28 @@ namespace dgops
               }
             for (int c = 0; c < 3; ++c)
               {
-                UV[c * m2 + t]       = u[c];
-                UV[(3 + c) * m2 + t] = v[c];
+                UV[c * m2 + r]       = u[c];
+                UV[(3 + c) * m2 + r] = v[c];
               }
           }
         __syncthreads();
         // integrate along x: F[c 3 + branch][qy n + i], branches d = x (G),
         // y (S), z (S)
-        for (int t = r; t < n * m; t += R)
+        if (t_nm)
           {
-            const int i = t % n, qy = t / n;
-            double    a[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-#pragma unroll 2
+            double a[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            const double *uv = UV + gq * m;
+#pragma unroll UM
             for (int qx = 0; qx < m; ++qx)
               {
-                const int    pt = qy * m + qx;
-                const double g  = sG[qx * n + i], sv = sS[qx * n + i];
-                const double c0 = g * UV[3 * m2 + pt];
-                const double c1 = sv * UV[4 * m2 + pt];
-                const double c2 = sv * UV[5 * m2 + pt];
+                const double g  = sG[qx * n + gi], sv = sS[qx * n + gi];
+                const double c0 = g * uv[3 * m2 + qx];
+                const double c1 = sv * uv[4 * m2 + qx];
+                const double c2 = sv * uv[5 * m2 + qx];
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
                   {
-                    const double uc = UV[c * m2 + pt];
+                    const double uc = uv[c * m2 + qx];
                     a[c][0]         = fma(c0, uc, a[c][0]);
                     a[c][1]         = fma(c1, uc, a[c][1]);
                     a[c][2]         = fma(c2, uc, a[c][2]);
@@ -393,26 +420,25 @@ namespace dgops
             for (int c = 0; c < 3; ++c)
 #pragma unroll
               for (int b = 0; b < 3; ++b)
-                F[(c * 3 + b) * m * n + qy * n + i] = a[c][b];
+                F[(c * 3 + b) * m * n + gq * n + gi] = a[c][b];
           }
         __syncthreads();
         // integrate along y; accumulate the z integration of this level
-        if (r < n2)
+        if (t_n2)
           {
-            const int i = r % n, j = r / n;
 #pragma unroll
             for (int c = 0; c < 3; ++c)
               {
-                double xy = 0, z = 0;
-#pragma unroll 2
+                double        xy = 0, z = 0;
+                const double *fc = F + (c * 3) * m * n + hi;
+#pragma unroll UM
                 for (int qy = 0; qy < m; ++qy)
                   {
-                    const double s  = sS[qy * n + j];
-                    const double g  = sG[qy * n + j];
-                    const int    o  = qy * n + i;
-                    xy = fma(s, F[(c * 3) * m * n + o], xy);
-                    xy = fma(g, F[(c * 3 + 1) * m * n + o], xy);
-                    z  = fma(s, F[(c * 3 + 2) * m * n + o], z);
+                    const double s = sS[qy * n + hj];
+                    const double g = sG[qy * n + hj];
+                    xy = fma(s, fc[qy * n], xy);
+                    xy = fma(g, fc[m * n + qy * n], xy);
+                    z  = fma(s, fc[2 * m * n + qy * n], z);
                   }
 #pragma unroll
                 for (int k = 0; k < n; ++k)
