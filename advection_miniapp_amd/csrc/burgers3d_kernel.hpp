@@ -69,8 +69,7 @@ namespace dgops
     static constexpr int R  = burgers3_threads(n, m);
     // per-cell LDS offsets (doubles)
     static constexpr int oU  = 0;                // [3][n3] nodal input
-    static constexpr int oRF = oU + 3 * n3;      // [3][n3] face terms
-    static constexpr int oW  = oRF + 3 * n3;     // [3][m][n2]
+    static constexpr int oW  = oU + 3 * n3;      // [3][m][n2]
     static constexpr int oT  = oW + 3 * m * n2;  // [3][n][m]
     static constexpr int oUV = oT + 3 * n * m;   // [6][m2]
     static constexpr int oF  = oUV + 6 * m2;     // [9][m][n]
@@ -78,6 +77,8 @@ namespace dgops
     static constexpr int oNB = oX + 24;          // [6][3][n2] neighbour
                                                  // face layers
     static constexpr int size = oNB + 18 * n2;
+    // unroll of the loops over the m points of a line
+    static constexpr int UM = m <= 12 ? m : 4;
   };
 
   template <int n, int m, bool cartesian, int epi, int CPB>
@@ -99,7 +100,6 @@ namespace dgops
                          lc;
     const bool    active = cell < P.n_cells;
     double       *U  = lds + lc * L::size + L::oU;
-    double       *RF = lds + lc * L::size + L::oRF;
     double       *W  = lds + lc * L::size + L::oW;
     double       *T  = lds + lc * L::size + L::oT;
     double       *UV = lds + lc * L::size + L::oUV;
@@ -120,10 +120,7 @@ namespace dgops
         sx[e] = BT->xm[e];
       }
     for (int e = r; e < nvec; e += R)
-      {
-        U[e]  = active ? src[e] : 0.;
-        RF[e] = 0.;
-      }
+      U[e] = active ? src[e] : 0.;
     if (!cartesian)
       for (int e = r; e < 24; e += R)
         X[e] = active ? P.cell_vertices[cell * 24 + e] : 0.;
@@ -155,7 +152,7 @@ namespace dgops
 
     // ---- faces: own and neighbour traces at the m x m face points, the
     // Rusanov flux with this cell's outward normal, integrated onto the own
-    // face layer (RF). Both faces of a direction at once (their layers are
+    // face layer. Both faces of a direction at once (their layers are
     // disjoint); one instance per direction, so the strides are static. The
     // face buffers overlay the cell-term buffers W..F, unused until the
     // cell term: FT [side s][own/neighbour][c][b m + qa] (12 n m values),
@@ -164,10 +161,17 @@ namespace dgops
                   "face buffers overlay the cell-term buffers");
     double *FT  = W;
     double *FUV = W + 12 * n * m;
+    // z-line owners (thread r < n2: i = r % n, j = r / n) accumulate the
+    // face and cell terms of their line in registers
+    double acc[3][n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int k = 0; k < n; ++k)
+        acc[c][k] = 0;
     // per-thread tasks, fixed for every phase below: each phase has at most
     // R tasks (R >= n m, m^2, n^2), one per thread, and a task covers all
     // components and sides
-    constexpr int UM = m <= 12 ? m : 4; // unroll of the loops over m points
     const bool    t_nm = r < n * m, t_m2 = r < m2, t_n2 = r < n2;
     const int     xj = r / m, xq = r % m; // (j or b, qx or qa)
     const int     yq = r % m, yy = r / m; // (qa, qb) or (qx, qy)
@@ -271,25 +275,56 @@ namespace dgops
             {
               const double *fl = FUV + ((k / 3) * 6 + k % 3) * m2 + qa;
               double        v  = 0;
-#pragma unroll UM
+#pragma unroll L::UM
               for (int qb = 0; qb < m; ++qb)
                 v = fma(sS[qb * n + b], fl[qb * m], v);
               FT[k * n * m + b * m + qa] = v;
             }
         }
       __syncthreads();
-      // ... then along ta onto the own layer of side s, thread (a, b)
+      // ... then along ta onto the own layer of side s, by the z-line
+      // owners of its nodes, into their registers: z faces, every owner
+      // (i, j) its node at k = layer; x faces, the owners (layer, j) and
+      // y faces the owners (i, layer) their whole line
       if (t_n2)
         {
-          const int a = hi, b = hj;
+          const int i = hi, j = hj;
 #pragma unroll
-          for (int k = 0; k < 6; ++k)
+          for (int s = 0; s < 2; ++s)
             {
-              double v = 0;
-#pragma unroll UM
-              for (int qa = 0; qa < m; ++qa)
-                v = fma(sS[qa * n + a], FT[k * n * m + b * m + qa], v);
-              RF[(k % 3) * n3 + FI::node(a, b, k / 3 ? n - 1 : 0)] += v;
+              const int layer = s ? n - 1 : 0;
+              if constexpr (d == 2)
+                {
+#pragma unroll
+                  for (int c = 0; c < 3; ++c)
+                    {
+                      const double *ft = FT + (s * 3 + c) * n * m + j * m;
+                      double        v  = 0;
+#pragma unroll L::UM
+                      for (int qa = 0; qa < m; ++qa)
+                        v = fma(sS[qa * n + i], ft[qa], v);
+                      acc[c][layer] += v;
+                    }
+                }
+              else
+                {
+                  const bool owner = d == 0 ? i == layer : j == layer;
+                  const int  a     = d == 0 ? j : i;
+                  if (owner)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+#pragma unroll
+                      for (int b = 0; b < n; ++b)
+                        {
+                          const double *ft =
+                            FT + (s * 3 + c) * n * m + b * m;
+                          double v = 0;
+#pragma unroll L::UM
+                          for (int qa = 0; qa < m; ++qa)
+                            v = fma(sS[qa * n + a], ft[qa], v);
+                          acc[c][b] += v;
+                        }
+                }
             }
         }
       __syncthreads();
@@ -317,16 +352,6 @@ namespace dgops
             W[(c * m + qz) * n2 + ij] = v;
           }
       }
-    // z-line owners (thread r < n2: i = r % n, j = r / n) accumulate
-    double acc[3][n];
-    {
-      const int ij = r < n2 ? r : 0;
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int k = 0; k < n; ++k)
-          acc[c][k] = RF[c * n3 + k * n2 + ij];
-    }
     __syncthreads();
     double vol = 0, adiag[3] = {0, 0, 0};
     if constexpr (cartesian)
@@ -400,7 +425,7 @@ namespace dgops
           {
             double a[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
             const double *uv = UV + gq * m;
-#pragma unroll UM
+#pragma unroll L::UM
             for (int qx = 0; qx < m; ++qx)
               {
                 const double g  = sG[qx * n + gi], sv = sS[qx * n + gi];
@@ -431,7 +456,7 @@ namespace dgops
               {
                 double        xy = 0, z = 0;
                 const double *fc = F + (c * 3) * m * n + hi;
-#pragma unroll UM
+#pragma unroll L::UM
                 for (int qy = 0; qy < m; ++qy)
                   {
                     const double s = sS[qy * n + hj];
