@@ -69,8 +69,8 @@ namespace dgops
     static constexpr int R  = burgers3_threads(n, m);
     // per-cell LDS offsets (doubles)
     static constexpr int oU  = 0;                // [3][n3] nodal input
-    static constexpr int oW  = oU + 3 * n3;      // [3][m][n2]
-    static constexpr int oT  = oW + 3 * m * n2;  // [3][n][m]
+    static constexpr int oW  = oU + 3 * n3;      // [2][3][n2] two levels
+    static constexpr int oT  = oW + 6 * n2;      // [3][n][m]
     static constexpr int oUV = oT + 3 * n * m;   // [6][m2]
     static constexpr int oF  = oUV + 6 * m2;     // [9][m][n]
     static constexpr int oX  = oF + 9 * m * n;   // [8][3] corners
@@ -156,11 +156,11 @@ namespace dgops
     // disjoint); one instance per direction, so the strides are static. The
     // face buffers overlay the cell-term buffers W..F, unused until the
     // cell term: FT [side s][own/neighbour][c][b m + qa] (12 n m values),
-    // FUV [s][own/neighbour][c][qb m + qa] (12 m^2 values)
-    static_assert(12 * n * m + 12 * m2 <= L::oX - L::oW,
+    // FL [s][c][qb m + qa], the flux on the own side (6 m^2 values)
+    static_assert(12 * n * m + 6 * m2 <= L::oX - L::oW,
                   "face buffers overlay the cell-term buffers");
     double *FT  = W;
-    double *FUV = W + 12 * n * m;
+    double *FL  = W + 12 * n * m;
     // z-line owners (thread r < n2: i = r % n, j = r / n) accumulate the
     // face and cell terms of their line in registers
     double acc[3][n];
@@ -202,10 +202,12 @@ namespace dgops
             }
         }
       __syncthreads();
-      // along tb: thread (qa, qb)
+      // along tb, then the flux, at the face point of thread (qa, qb),
+      // both sides; FL receives -F n dA of the own side
       if (t_m2)
         {
-          const int qa = yq, qb = yy;
+          const int qa = yq, qb = yy, pt = r;
+          double    tr[12]; // [s][own/neighbour][c]
 #pragma unroll
           for (int k = 0; k < 12; ++k)
             {
@@ -213,16 +215,9 @@ namespace dgops
 #pragma unroll
               for (int b = 0; b < n; ++b)
                 v = fma(sS[qb * n + b], FT[k * n * m + b * m + qa], v);
-              FUV[k * m2 + qb * m + qa] = v;
+              tr[k] = v;
             }
-        }
-      __syncthreads();
-      // flux at the face point of thread (qa, qb), both sides; the own
-      // slots receive -F n dA
-      if (t_m2)
-        {
-          const int    qa = yq, qb = yy, pt = r;
-          const double w  = sw[qa] * sw[qb];
+          const double w = sw[qa] * sw[qb];
 #pragma unroll
           for (int s = 0; s < 2; ++s)
             {
@@ -249,18 +244,16 @@ namespace dgops
                 }
               const double len =
                 sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
-              double *own = FUV + s * 6 * m2, *nbr = own + 3 * m2;
-              double  um[3], up[3], unm = 0, unp = 0;
+              const double *um = tr + 6 * s, *up = um + 3;
+              double        unm = 0, unp = 0;
               for (int c = 0; c < 3; ++c)
                 {
-                  um[c] = own[c * m2 + pt];
-                  up[c] = nbr[c * m2 + pt];
                   unm += um[c] * nrm[c] / len;
                   unp += up[c] * nrm[c] / len;
                 }
               const double lam = P.alpha * fmax(fabs(unm), fabs(unp));
               for (int c = 0; c < 3; ++c)
-                own[c * m2 + pt] =
+                FL[(s * 3 + c) * m2 + pt] =
                   -0.5 * (unm * um[c] + unp * up[c] + lam * (um[c] - up[c])) *
                   len;
             }
@@ -273,7 +266,7 @@ namespace dgops
 #pragma unroll
           for (int k = 0; k < 6; ++k)
             {
-              const double *fl = FUV + ((k / 3) * 6 + k % 3) * m2 + qa;
+              const double *fl = FL + k * m2 + qa;
               double        v  = 0;
 #pragma unroll L::UM
               for (int qb = 0; qb < m; ++qb)
@@ -334,23 +327,30 @@ namespace dgops
     face_pair(FaceId<2>{});
 
     // ---- cell term, slab by slab ----------------------------------------
-    // W[c][qz][j n + i] = sum_k S[qz][k] u[c][k][j][i]
-    for (int t = r; t < 3 * n2; t += R)
+    // level qz of W[qz % 2][c][j n + i] = sum_k S[qz][k] u[c][k][j][i],
+    // formed by the z-line owners from their nodal line (uz) one level
+    // ahead, during the y phase of the level before
+    double uz[3][n];
+    auto   w_level = [&](int qz) {
+      double *w = W + (qz % 2) * 3 * n2 + r;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        {
+          double v = 0;
+#pragma unroll
+          for (int k = 0; k < n; ++k)
+            v = fma(sS[qz * n + k], uz[c][k], v);
+          w[c * n2] = v;
+        }
+    };
+    if (t_n2)
       {
-        const int c = t / n2, ij = t % n2;
-        double    u[n];
 #pragma unroll
-        for (int k = 0; k < n; ++k)
-          u[k] = U[c * n3 + k * n2 + ij];
-#pragma unroll 2
-        for (int qz = 0; qz < m; ++qz)
-          {
-            double v = 0;
+        for (int c = 0; c < 3; ++c)
 #pragma unroll
-            for (int k = 0; k < n; ++k)
-              v = fma(sS[qz * n + k], u[k], v);
-            W[(c * m + qz) * n2 + ij] = v;
-          }
+          for (int k = 0; k < n; ++k)
+            uz[c][k] = U[c * n3 + k * n2 + r];
+        w_level(0);
       }
     __syncthreads();
     double vol = 0, adiag[3] = {0, 0, 0};
@@ -369,18 +369,22 @@ namespace dgops
         // along x: T[c][j m + qx]
         if (t_nm)
           {
-            const double *w = W + qz * n2 + xj * n;
+            const double *w = W + (qz % 2) * 3 * n2 + xj * n;
 #pragma unroll
             for (int c = 0; c < 3; ++c)
               {
                 double v = 0;
 #pragma unroll
                 for (int i = 0; i < n; ++i)
-                  v = fma(sS[xq * n + i], w[c * m * n2 + i], v);
+                  v = fma(sS[xq * n + i], w[c * n2 + i], v);
                 T[c * n * m + xj * m + xq] = v;
               }
           }
         __syncthreads();
+        // the next level of W (its buffer was last read by the x phase of
+        // the level before)
+        if (t_n2 && qz + 1 < m)
+          w_level(qz + 1);
         // along y, then the point-wise fluxes: UV = (u_0, u_1, u_2, v_0,
         // v_1, v_2) with v = w adj(J) u
         if (t_m2)
