@@ -48,8 +48,8 @@ namespace dgops
   // with (M2, in2)
   template <int dim, int n, int dir, bool trans, bool two>
   __device__ __forceinline__ void
-  sipg_sweep(const double (&M)[max_n][max_n], const double *in,
-             const double (&M2)[max_n][max_n], const double *in2, double *out)
+  sipg_sweep(const double *M, const double *in, const double *M2,
+             const double *in2, double *out)
   {
     using L           = SipgLay<dim, n>;
     constexpr int st  = L::stride(dir);
@@ -61,16 +61,23 @@ namespace dgops
 #pragma unroll
         for (int i = 0; i < n; ++i)
           {
-            s = fma(trans ? M[i][r] : M[r][i], in[base + i * st], s);
+            s = fma(trans ? M[i * n + r] : M[r * n + i], in[base + i * st], s);
             if constexpr (two)
-              s = fma(trans ? M2[i][r] : M2[r][i], in2[base + i * st], s);
+              s = fma(trans ? M2[i * n + r] : M2[r * n + i], in2[base + i * st],
+                      s);
           }
         out[p] = s;
       }
   }
 
+  template <int d>
+  struct FaceDir
+  {
+    static constexpr int value = d;
+  };
+
   template <int dim, int n>
-  __global__ void __launch_bounds__(128)
+  __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
   sipg_general_kernel(const OpParams                       P,
                       const DevBasis<double> *__restrict__ Bp,
                       const double *__restrict__           src,
@@ -79,10 +86,15 @@ namespace dgops
   {
     using L          = SipgLay<dim, n>;
     constexpr int nd = L::nd, nf = L::nf, NV = L::nv;
-    __shared__ double su[nd], sn[nd], A[6][nd], sres[nd];
-    __shared__ double sX[NV * dim], sXn[NV * dim];
-    __shared__ double fw[6][nf]; // tangential partial sums, own | neighbour
-    __shared__ double fc[4][nf]; // face coefficients: value, d, t1, t2
+    __shared__ double su[nd], sn[2][nd], A[6][nd], sres[nd];
+    __shared__ double sX[NV * dim], sXn[2][NV * dim];
+    // per face s of the direction: tangential partial sums [own |
+    // neighbour][3 nf], and the face coefficients (value, d, t1, t2)
+    __shared__ double fw[2][2][3 * nf];
+    __shared__ double fc[2][4][nf];
+    // the 1D tables in LDS (row stride n): S and G at the Gauss points,
+    // and the nodal-basis derivatives at the two faces
+    __shared__ double tS[n * n], tG[n * n], tD[2][n];
 
     const DevBasis<double> &B    = *Bp;
     const int64_t           cell = blockIdx.x;
@@ -97,33 +109,40 @@ namespace dgops
       }
     for (int e = tid; e < NV * dim; e += blockDim.x)
       sX[e] = P.cell_vertices[cell * NV * dim + e];
+    for (int e = tid; e < n * n; e += blockDim.x)
+      {
+        tS[e] = B.S[e / n][e % n];
+        tG[e] = B.G[e / n][e % n];
+      }
+    for (int e = tid; e < 2 * n; e += blockDim.x)
+      tD[e / n][e % n] = e < n ? B.dphi0[e] : B.dphi1[e - n];
     __syncthreads();
 
     // ---- cell term: reference gradients at the Gauss points ----
     int gidx[3]; // A[] slot of the reference gradient along each direction
     if constexpr (dim == 3)
       {
-        sipg_sweep<3, n, 0, false, false>(B.S, su, B.S, su, A[0]);
-        sipg_sweep<3, n, 0, false, false>(B.G, su, B.G, su, A[1]);
+        sipg_sweep<3, n, 0, false, false>(tS, su, tS, su, A[0]);
+        sipg_sweep<3, n, 0, false, false>(tG, su, tG, su, A[1]);
         __syncthreads();
-        sipg_sweep<3, n, 1, false, false>(B.S, A[0], B.S, A[0], A[2]);
-        sipg_sweep<3, n, 1, false, false>(B.S, A[1], B.S, A[1], A[3]);
-        sipg_sweep<3, n, 1, false, false>(B.G, A[0], B.G, A[0], A[4]);
+        sipg_sweep<3, n, 1, false, false>(tS, A[0], tS, A[0], A[2]);
+        sipg_sweep<3, n, 1, false, false>(tS, A[1], tS, A[1], A[3]);
+        sipg_sweep<3, n, 1, false, false>(tG, A[0], tG, A[0], A[4]);
         __syncthreads();
-        sipg_sweep<3, n, 2, false, false>(B.G, A[2], B.G, A[2], A[5]);
-        sipg_sweep<3, n, 2, false, false>(B.S, A[3], B.S, A[3], A[0]);
-        sipg_sweep<3, n, 2, false, false>(B.S, A[4], B.S, A[4], A[1]);
+        sipg_sweep<3, n, 2, false, false>(tG, A[2], tG, A[2], A[5]);
+        sipg_sweep<3, n, 2, false, false>(tS, A[3], tS, A[3], A[0]);
+        sipg_sweep<3, n, 2, false, false>(tS, A[4], tS, A[4], A[1]);
         gidx[0] = 0;
         gidx[1] = 1;
         gidx[2] = 5;
       }
     else
       {
-        sipg_sweep<2, n, 0, false, false>(B.S, su, B.S, su, A[0]);
-        sipg_sweep<2, n, 0, false, false>(B.G, su, B.G, su, A[1]);
+        sipg_sweep<2, n, 0, false, false>(tS, su, tS, su, A[0]);
+        sipg_sweep<2, n, 0, false, false>(tG, su, tG, su, A[1]);
         __syncthreads();
-        sipg_sweep<2, n, 1, false, false>(B.S, A[1], B.S, A[1], A[2]);
-        sipg_sweep<2, n, 1, false, false>(B.G, A[0], B.G, A[0], A[3]);
+        sipg_sweep<2, n, 1, false, false>(tS, A[1], tS, A[1], A[2]);
+        sipg_sweep<2, n, 1, false, false>(tG, A[0], tG, A[0], A[3]);
         gidx[0] = 2;
         gidx[1] = 3;
         gidx[2] = 3;
@@ -164,214 +183,233 @@ namespace dgops
     // integration with the test-function gradients
     if constexpr (dim == 3)
       {
-        sipg_sweep<3, n, 2, true, false>(B.S, A[0], B.S, A[0], A[2]);
-        sipg_sweep<3, n, 2, true, false>(B.S, A[1], B.S, A[1], A[3]);
-        sipg_sweep<3, n, 2, true, false>(B.G, A[5], B.G, A[5], A[4]);
+        sipg_sweep<3, n, 2, true, false>(tS, A[0], tS, A[0], A[2]);
+        sipg_sweep<3, n, 2, true, false>(tS, A[1], tS, A[1], A[3]);
+        sipg_sweep<3, n, 2, true, false>(tG, A[5], tG, A[5], A[4]);
         __syncthreads();
-        sipg_sweep<3, n, 1, true, false>(B.S, A[2], B.S, A[2], A[0]);
-        sipg_sweep<3, n, 1, true, true>(B.G, A[3], B.S, A[4], A[1]);
+        sipg_sweep<3, n, 1, true, false>(tS, A[2], tS, A[2], A[0]);
+        sipg_sweep<3, n, 1, true, true>(tG, A[3], tS, A[4], A[1]);
         __syncthreads();
-        sipg_sweep<3, n, 0, true, true>(B.G, A[0], B.S, A[1], sres);
+        sipg_sweep<3, n, 0, true, true>(tG, A[0], tS, A[1], sres);
       }
     else
       {
-        sipg_sweep<2, n, 1, true, false>(B.S, A[2], B.S, A[2], A[0]);
-        sipg_sweep<2, n, 1, true, false>(B.G, A[3], B.G, A[3], A[1]);
+        sipg_sweep<2, n, 1, true, false>(tS, A[2], tS, A[2], A[0]);
+        sipg_sweep<2, n, 1, true, false>(tG, A[3], tG, A[3], A[1]);
         __syncthreads();
-        sipg_sweep<2, n, 0, true, true>(B.G, A[0], B.S, A[1], sres);
+        sipg_sweep<2, n, 0, true, true>(tG, A[0], tS, A[1], sres);
       }
     __syncthreads();
 
-    // ---- faces ----
-    for (int f = 0; f < 2 * dim; ++f)
-      {
-        const int d = f / 2, s = f % 2;
-        const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
-        const int nb = P.neighbors[cell * 2 * dim + f];
-        if (nb < 0) // SIPG runs on periodic meshes; no boundary terms
-          continue;
-        for (int p = tid; p < nd; p += blockDim.x)
-          sn[p] = src[(int64_t)nb * nd + p];
-        for (int e = tid; e < NV * dim; e += blockDim.x)
-          sXn[e] = P.cell_vertices[(int64_t)nb * NV * dim + e];
-        __syncthreads();
-        const int sd = L::stride(d), s1 = L::stride(t1),
-                  s2 = dim == 3 ? L::stride(t2) : 0;
-        // own layer at side s, neighbour layer at side 1 - s: values are the
-        // layer's nodes (Gauss-Lobatto), normal derivatives dphi . line;
-        // first tangential direction: fw[0..2] own (S v, G v, S dv),
-        // fw[3..5] neighbour
-        for (int t = tid; t < 2 * nf; t += blockDim.x)
-          {
-            const int     side = t / nf, pt = t % nf;
-            const int     a = pt % n, j = dim == 3 ? pt / n : 0;
-            const double *u    = side ? sn : su;
-            const int     ls   = side ? 1 - s : s;
-            const int     lay  = ls ? n - 1 : 0;
-            const double(&dp)[max_n] = ls ? B.dphi1 : B.dphi0;
-            double v = 0, vg = 0, dv = 0;
-#pragma unroll
-            for (int i = 0; i < n; ++i)
-              {
-                const int base = i * s1 + j * s2;
-                const double ui = u[base + lay * sd];
-                double       di = 0;
-#pragma unroll
-                for (int l = 0; l < n; ++l)
-                  di = fma(dp[l], u[base + l * sd], di);
-                v  = fma(B.S[a][i], ui, v);
-                vg = fma(B.G[a][i], ui, vg);
-                dv = fma(B.S[a][i], di, dv);
-              }
-            fw[3 * side][pt]     = v;
-            fw[3 * side + 1][pt] = vg;
-            fw[3 * side + 2][pt] = dv;
-          }
-        __syncthreads();
-        // second tangential direction, geometry and flux per face point
-        for (int pt = tid; pt < nf; pt += blockDim.x)
-          {
-            const int a = pt % n, b = dim == 3 ? pt / n : 0;
-            double    val[2], gref[2][3];
-#pragma unroll
-            for (int side = 0; side < 2; ++side)
-              {
-                double v = 0, g1 = 0, g2 = 0, gd = 0;
-                if constexpr (dim == 3)
-                  {
-#pragma unroll
-                    for (int jj = 0; jj < n; ++jj)
-                      {
-                        const int o = jj * n + a;
-                        v  = fma(B.S[b][jj], fw[3 * side][o], v);
-                        g1 = fma(B.S[b][jj], fw[3 * side + 1][o], g1);
-                        g2 = fma(B.G[b][jj], fw[3 * side][o], g2);
-                        gd = fma(B.S[b][jj], fw[3 * side + 2][o], gd);
-                      }
-                  }
-                else
-                  {
-                    v  = fw[3 * side][a];
-                    g1 = fw[3 * side + 1][a];
-                    gd = fw[3 * side + 2][a];
-                  }
-                val[side]         = v;
-                gref[side][d]     = gd;
-                gref[side][t1]    = g1;
-                if constexpr (dim == 3)
-                  gref[side][t2] = g2;
-              }
-            double xi[3];
-            xi[d]  = s;
-            xi[t1] = B.xq[a];
-            if constexpr (dim == 3)
-              xi[t2] = B.xq[b];
-            double x[3], J[3][3], adj[3][3], Jn[3][3], adjn[3][3];
-            multilinear_map<dim>(sX, xi, x, J);
-            const double det = adjugate<dim>(J, adj);
-            xi[d] = 1 - s;
-            multilinear_map<dim>(sXn, xi, x, Jn);
-            const double detn = adjugate<dim>(Jn, adjn);
-            // area vector of the face (row d of adj J, along +xi_d)
-            double len = 0;
-#pragma unroll
-            for (int e = 0; e < dim; ++e)
-              len = fma(adj[d][e], adj[d][e], len);
-            len = sqrt(len);
-            double nrm[3] = {0, 0, 0};
-#pragma unroll
-            for (int e = 0; e < dim; ++e)
-              nrm[e] = (s ? 1. : -1.) * adj[d][e] / len;
-            double JxW = len * B.w[a];
-            if constexpr (dim == 3)
-              JxW *= B.w[b];
-            // J^-1 n on both sides and the normal derivatives
-            double jn[3] = {0, 0, 0}, jnn[3] = {0, 0, 0};
-#pragma unroll
-            for (int k = 0; k < dim; ++k)
-              {
-#pragma unroll
-                for (int e = 0; e < dim; ++e)
-                  {
-                    jn[k]  = fma(adj[k][e], nrm[e], jn[k]);
-                    jnn[k] = fma(adjn[k][e], nrm[e], jnn[k]);
-                  }
-                jn[k] /= det;
-                jnn[k] /= detn;
-              }
-            double dnm = 0, dnp = 0;
-#pragma unroll
-            for (int k = 0; k < dim; ++k)
-              {
-                dnm = fma(jn[k], gref[0][k], dnm);
-                dnp = fma(jnn[k], gref[1][k], dnp);
-              }
-            const double sigma =
-              P.penalty * 0.5 * (fabs(jn[d]) + fabs(jnn[d]));
-            const double jump    = -0.5 * nu * (val[0] - val[1]);
-            const double avg     = -0.5 * nu * (dnm + dnp);
-            const double viscous = jump * sigma - avg;
-            const double cd      = sg * jump * JxW;
-            fc[0][pt] = -sg * viscous * JxW;
-            fc[1][pt] = cd * jn[d];
-            fc[2][pt] = cd * jn[t1];
-            fc[3][pt] = dim == 3 ? cd * jn[t2] : 0.;
-          }
-        __syncthreads();
-        // test functions: along t2 (transposed) into fw, then along t1 and
-        // onto the nodes; phi_i(s) picks the layer (Gauss-Lobatto nodes),
-        // dphi_s carries the normal-derivative coefficient
-        if constexpr (dim == 3)
-          {
-            for (int t = tid; t < nf; t += blockDim.x)
-              {
-                const int a = t % n, i2 = t / n;
-                double    z0 = 0, z1 = 0, z2 = 0, z3 = 0;
-#pragma unroll
-                for (int b = 0; b < n; ++b)
-                  {
-                    const int o = b * n + a;
-                    z0 = fma(B.S[b][i2], fc[0][o], z0);
-                    z1 = fma(B.S[b][i2], fc[2][o], z1);
-                    z2 = fma(B.G[b][i2], fc[3][o], z2);
-                    z3 = fma(B.S[b][i2], fc[1][o], z3);
-                  }
-                fw[0][i2 * n + a] = z0 + z2;
-                fw[1][i2 * n + a] = z1;
-                fw[2][i2 * n + a] = z3;
-              }
-            __syncthreads();
-          }
+    // ---- faces: both faces of a direction per pass (direction d a
+    // compile-time constant, so every index below is static) ----
+    auto face_pair = [&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      constexpr int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+      constexpr int sd = L::stride(d), s1 = L::stride(t1),
+                    s2 = dim == 3 ? L::stride(t2) : 0;
+      // neighbours across the two faces (periodic meshes: both exist)
+      const int nbs[2] = {P.neighbors[cell * 2 * dim + 2 * d],
+                          P.neighbors[cell * 2 * dim + 2 * d + 1]};
+      for (int p = tid; p < 2 * nd; p += blockDim.x)
         {
-          const int      lay = s ? n - 1 : 0;
-          const double(&dp)[max_n] = s ? B.dphi1 : B.dphi0;
-          for (int p = tid; p < nd; p += blockDim.x)
+          const int s = p / nd, q = p - s * nd;
+          sn[s][q]    = nbs[s] >= 0 ? src[(int64_t)nbs[s] * nd + q] : 0.;
+        }
+      for (int e = tid; e < 2 * NV * dim; e += blockDim.x)
+        {
+          const int s = e / (NV * dim), q = e - s * NV * dim;
+          sXn[s][q]   = nbs[s] >= 0 ?
+                          P.cell_vertices[(int64_t)nbs[s] * NV * dim + q] :
+                          sX[q];
+        }
+      __syncthreads();
+      // layer values and normal derivatives, first tangential direction:
+      // fw[s][side][0..2] = (S v, G v, S dv) at (a, j); side 0 own (layer
+      // s), side 1 the neighbour across face s (its layer 1 - s)
+      for (int t = tid; t < 4 * nf; t += blockDim.x)
+        {
+          const int     s = t / (2 * nf), side = (t / nf) % 2, pt = t % nf;
+          const int     a = pt % n, j = dim == 3 ? pt / n : 0;
+          const double *u  = side ? sn[s] : su;
+          const int     ls = side ? 1 - s : s;
+          const int     lay = ls ? n - 1 : 0;
+          const double *dp = tD[ls];
+          double v = 0, vg = 0, dv = 0;
+#pragma unroll
+          for (int i = 0; i < n; ++i)
             {
-              const int id = (p / sd) % n, i1 = (p / s1) % n;
-              const int i2 = dim == 3 ? (p / s2) % n : 0;
-              double    v0 = 0, v1 = 0;
+              const int    base = i * s1 + j * s2;
+              const double ui   = u[base + lay * sd];
+              double       di   = 0;
+#pragma unroll
+              for (int l = 0; l < n; ++l)
+                di = fma(dp[l], u[base + l * sd], di);
+              v  = fma(tS[a * n + i], ui, v);
+              vg = fma(tG[a * n + i], ui, vg);
+              dv = fma(tS[a * n + i], di, dv);
+            }
+          double *o = fw[s][side];
+          o[pt]          = v;
+          o[nf + pt]     = vg;
+          o[2 * nf + pt] = dv;
+        }
+      __syncthreads();
+      // second tangential direction, geometry and flux per face point
+      for (int t = tid; t < 2 * nf; t += blockDim.x)
+        {
+          const int s = t / nf, pt = t % nf;
+          const int a = pt % n, b = dim == 3 ? pt / n : 0;
+          double    val[2], gref[2][3];
+#pragma unroll
+          for (int side = 0; side < 2; ++side)
+            {
+              const double *o = fw[s][side];
+              double        v = 0, g1 = 0, g2 = 0, gd = 0;
+              if constexpr (dim == 3)
+                {
+#pragma unroll
+                  for (int jj = 0; jj < n; ++jj)
+                    {
+                      const int q = jj * n + a;
+                      v  = fma(tS[b * n + jj], o[q], v);
+                      g1 = fma(tS[b * n + jj], o[nf + q], g1);
+                      g2 = fma(tG[b * n + jj], o[q], g2);
+                      gd = fma(tS[b * n + jj], o[2 * nf + q], gd);
+                    }
+                }
+              else
+                {
+                  v  = o[a];
+                  g1 = o[nf + a];
+                  gd = o[2 * nf + a];
+                }
+              val[side]      = v;
+              gref[side][d]  = gd;
+              gref[side][t1] = g1;
+              if constexpr (dim == 3)
+                gref[side][t2] = g2;
+            }
+          double xi[3] = {0, 0, 0};
+          xi[t1] = B.xq[a];
+          if constexpr (dim == 3)
+            xi[t2] = B.xq[b];
+          double x[3], J[3][3], adj[3][3], Jn[3][3], adjn[3][3];
+          xi[d] = s;
+          multilinear_map<dim>(sX, xi, x, J);
+          const double det = adjugate<dim>(J, adj);
+          xi[d] = 1 - s;
+          multilinear_map<dim>(sXn[s], xi, x, Jn);
+          const double detn = adjugate<dim>(Jn, adjn);
+          // area vector of the face (row d of adj J, along +xi_d)
+          double len = 0;
+#pragma unroll
+          for (int e = 0; e < dim; ++e)
+            len = fma(adj[d][e], adj[d][e], len);
+          len = sqrt(len);
+          double nrm[3] = {0, 0, 0};
+#pragma unroll
+          for (int e = 0; e < dim; ++e)
+            nrm[e] = (s ? 1. : -1.) * adj[d][e] / len;
+          double JxW = len * B.w[a];
+          if constexpr (dim == 3)
+            JxW *= B.w[b];
+          // J^-1 n on both sides and the normal derivatives
+          double jn[3] = {0, 0, 0}, jnn[3] = {0, 0, 0};
+#pragma unroll
+          for (int k = 0; k < dim; ++k)
+            {
+#pragma unroll
+              for (int e = 0; e < dim; ++e)
+                {
+                  jn[k]  = fma(adj[k][e], nrm[e], jn[k]);
+                  jnn[k] = fma(adjn[k][e], nrm[e], jnn[k]);
+                }
+              jn[k] /= det;
+              jnn[k] /= detn;
+            }
+          double dnm = 0, dnp = 0;
+#pragma unroll
+          for (int k = 0; k < dim; ++k)
+            {
+              dnm = fma(jn[k], gref[0][k], dnm);
+              dnp = fma(jnn[k], gref[1][k], dnp);
+            }
+          const double sigma   = P.penalty * 0.5 * (fabs(jn[d]) + fabs(jnn[d]));
+          const double jump    = -0.5 * nu * (val[0] - val[1]);
+          const double avg     = -0.5 * nu * (dnm + dnp);
+          const double viscous = jump * sigma - avg;
+          const double cd      = sg * jump * JxW;
+          fc[s][0][pt] = -sg * viscous * JxW;
+          fc[s][1][pt] = cd * jn[d];
+          fc[s][2][pt] = cd * jn[t1];
+          fc[s][3][pt] = dim == 3 ? cd * jn[t2] : 0.;
+        }
+      __syncthreads();
+      // test functions: along t2 (transposed) into fw[s][0], then along t1
+      // and onto the nodes; phi_i(s) picks the layer (Gauss-Lobatto nodes),
+      // dphi_s carries the normal-derivative coefficient
+      if constexpr (dim == 3)
+        {
+          for (int t = tid; t < 2 * nf; t += blockDim.x)
+            {
+              const int s = t / nf, pt = t % nf;
+              const int a = pt % n, i2 = pt / n;
+              double    z0 = 0, z1 = 0, z2 = 0, z3 = 0;
+#pragma unroll
+              for (int b = 0; b < n; ++b)
+                {
+                  const int o = b * n + a;
+                  z0 = fma(tS[b * n + i2], fc[s][0][o], z0);
+                  z1 = fma(tS[b * n + i2], fc[s][2][o], z1);
+                  z2 = fma(tG[b * n + i2], fc[s][3][o], z2);
+                  z3 = fma(tS[b * n + i2], fc[s][1][o], z3);
+                }
+              double *o = fw[s][0];
+              o[pt]          = z0 + z2;
+              o[nf + pt]     = z1;
+              o[2 * nf + pt] = z3;
+            }
+          __syncthreads();
+        }
+      for (int p = tid; p < nd; p += blockDim.x)
+        {
+          const int id = (p / sd) % n, i1 = (p / s1) % n;
+          const int i2 = dim == 3 ? (p / s2) % n : 0;
+          double    add = 0;
+#pragma unroll
+          for (int s = 0; s < 2; ++s)
+            {
+              const double *dp = tD[s];
+              double v0 = 0, v1 = 0;
 #pragma unroll
               for (int a = 0; a < n; ++a)
                 {
                   if constexpr (dim == 3)
                     {
-                      const int o = i2 * n + a;
-                      v0 = fma(B.S[a][i1], fw[0][o],
-                               fma(B.G[a][i1], fw[1][o], v0));
-                      v1 = fma(B.S[a][i1], fw[2][o], v1);
+                      const double *o = fw[s][0];
+                      const int     q = i2 * n + a;
+                      v0 = fma(tS[a * n + i1], o[q],
+                               fma(tG[a * n + i1], o[nf + q], v0));
+                      v1 = fma(tS[a * n + i1], o[2 * nf + q], v1);
                     }
                   else
                     {
-                      v0 = fma(B.S[a][i1], fc[0][a],
-                               fma(B.G[a][i1], fc[2][a], v0));
-                      v1 = fma(B.S[a][i1], fc[1][a], v1);
+                      v0 = fma(tS[a * n + i1], fc[s][0][a],
+                               fma(tG[a * n + i1], fc[s][2][a], v0));
+                      v1 = fma(tS[a * n + i1], fc[s][1][a], v1);
                     }
                 }
-              sres[p] += (id == lay ? v0 : 0.) + dp[id] * v1;
+              add += (id == (s ? n - 1 : 0) ? v0 : 0.) + dp[id] * v1;
             }
+          sres[p] += add;
         }
-        __syncthreads();
-      }
+      __syncthreads();
+    };
+    face_pair(FaceDir<0>{});
+    face_pair(FaceDir<1>{});
+    if constexpr (dim == 3)
+      face_pair(FaceDir<2>{});
     for (int p = tid; p < nd; p += blockDim.x)
       dst[cell * nd + p] += scale * sres[p];
   }
