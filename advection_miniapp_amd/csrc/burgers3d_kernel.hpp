@@ -74,15 +74,16 @@ namespace dgops
     static constexpr int oUV = oT + 3 * n * m;   // [6][m2]
     static constexpr int oF  = oUV + 6 * m2;     // [9][m][n]
     static constexpr int oX  = oF + 9 * m * n;   // [8][3] corners
-    static constexpr int oNB = oX + 24;          // [6][3][n2] neighbour
-                                                 // face layers
-    static constexpr int size = oNB + 18 * n2;
+    static constexpr int size = oX + 24;
     // unroll of the loops over the m points of a line
     static constexpr int UM = m <= 12 ? m : 4;
   };
 
+  // four wavefronts per SIMD where that costs no spills (m <= 6, degree
+  // <= 3: 128 VGPRs with 20 KB of LDS per 3-cell workgroup at degree 3)
   template <int n, int m, bool cartesian, int epi, int CPB>
   __global__ void __launch_bounds__(CPB * burgers3_threads(n, m))
+  __attribute__((amdgpu_waves_per_eu(m <= 6 ? 4 : 1)))
   burgers3_kernel(const OpParams P, const DevBasis<double> *__restrict__ Bd,
                   const BurgersTables *__restrict__ BT,
                   const OpBuffers<double> io)
@@ -105,7 +106,6 @@ namespace dgops
     double       *UV = lds + lc * L::size + L::oUV;
     double       *F  = lds + lc * L::size + L::oF;
     double       *X  = lds + lc * L::size + L::oX;
-    double       *NB = lds + lc * L::size + L::oNB;
     const double *src = io.src + cell * nvec;
 
     // ---- inputs: nodal values, corners, the QGauss(m) tables -----------
@@ -124,30 +124,6 @@ namespace dgops
     if (!cartesian)
       for (int e = r; e < 24; e += R)
         X[e] = active ? P.cell_vertices[cell * 24 + e] : 0.;
-    // the neighbours' layers facing this cell, all six faces before the
-    // first barrier: one round trip instead of one per face.
-    // NB[(f 3 + c) n2 + b n + a] is node (a, b) of face f (a along the first,
-    // b along the second tangential direction) of component c
-    auto load_nb = [&](auto fc) {
-      constexpr int f  = decltype(fc)::value;
-      using FI         = FaceIdx<n, f>;
-      const int64_t nb =
-        active ? (int64_t)P.neighbors[cell * 6 + f] : (int64_t)-1;
-      for (int e = r; e < 3 * n2; e += R)
-        {
-          const int c = e / n2, a = e % n, b = (e / n) % n;
-          NB[(f * 3 + c) * n2 + b * n + a] =
-            nb >= 0 ?
-              io.src[nb * nvec + c * n3 + FI::node(a, b, FI::s ? 0 : n - 1)] :
-              0.;
-        }
-    };
-    load_nb(FaceId<0>{});
-    load_nb(FaceId<1>{});
-    load_nb(FaceId<2>{});
-    load_nb(FaceId<3>{});
-    load_nb(FaceId<4>{});
-    load_nb(FaceId<5>{});
     __syncthreads();
 
     // ---- faces: own and neighbour traces at the m x m face points, the
@@ -185,6 +161,23 @@ namespace dgops
       if (t_nm)
         {
           const int b = xj, qa = xq;
+          // row b of the neighbours' layers facing this cell (face s: the
+          // neighbour's layer 1 - s), from global memory
+          double nv[2][3][n];
+#pragma unroll
+          for (int s = 0; s < 2; ++s)
+            {
+              const int64_t nb =
+                active ? (int64_t)P.neighbors[cell * 6 + 2 * d + s] : -1;
+#pragma unroll
+              for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int a = 0; a < n; ++a)
+                  nv[s][c][a] =
+                    nb >= 0 ? io.src[nb * nvec + c * n3 +
+                                     FI::node(a, b, s ? 0 : n - 1)] :
+                              0.;
+            }
 #pragma unroll
           for (int k = 0; k < 12; ++k)
             {
@@ -195,7 +188,7 @@ namespace dgops
                 {
                   const double x =
                     sd == 0 ? U[c * n3 + FI::node(a, b, s ? n - 1 : 0)] :
-                              NB[((2 * d + s) * 3 + c) * n2 + b * n + a];
+                              nv[s][c][a];
                   v = fma(sS[qa * n + a], x, v);
                 }
               FT[k * n * m + b * m + qa] = v;
@@ -328,10 +321,9 @@ namespace dgops
 
     // ---- cell term, slab by slab ----------------------------------------
     // level qz of W[qz % 2][c][j n + i] = sum_k S[qz][k] u[c][k][j][i],
-    // formed by the z-line owners from their nodal line (uz) one level
+    // formed by the z-line owners from their nodal line (in U) one level
     // ahead, during the y phase of the level before
-    double uz[3][n];
-    auto   w_level = [&](int qz) {
+    auto w_level = [&](int qz) {
       double *w = W + (qz % 2) * 3 * n2 + r;
 #pragma unroll
       for (int c = 0; c < 3; ++c)
@@ -339,19 +331,12 @@ namespace dgops
           double v = 0;
 #pragma unroll
           for (int k = 0; k < n; ++k)
-            v = fma(sS[qz * n + k], uz[c][k], v);
+            v = fma(sS[qz * n + k], U[c * n3 + k * n2 + r], v);
           w[c * n2] = v;
         }
     };
     if (t_n2)
-      {
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-          for (int k = 0; k < n; ++k)
-            uz[c][k] = U[c * n3 + k * n2 + r];
-        w_level(0);
-      }
+      w_level(0);
     __syncthreads();
     double vol = 0, adiag[3] = {0, 0, 0};
     if constexpr (cartesian)
