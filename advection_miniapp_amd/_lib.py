@@ -111,6 +111,7 @@ SIGNATURES = {
     "dg_apply": (_i, [_vp, _vp, _vp, _d]),
     "dg_perform_stage": (_i, [_vp, _d, _d, _d, _vp, _vp, _vp, _vp]),
     "dg_lsrk_time_step": (_i, [_vp, _i, _d, _d, _vp, _vp, _vp]),
+    "dg_lsrk_time_steps": (_i, [_vp, _i, _d, _d, _i, _vp, _vp, _vp]),
     "dg_lsrk_n_stages": (_i, [_i]),
     "dg_lsrk_coefficients": (_i, [_i, _dp, _dp]),
     "dg_project_initial": (_i, [_vp, _vp, _d]),

@@ -55,7 +55,7 @@ namespace dgops
       burgers_key(op.cfg.dim, op.n, P.cartesian ? 1 : 0, epi));
     if (it == burgers_registry().end())
       throw Error(DG_ERR_UNSUPPORTED,
-                  "Burgers operator built for degree 1..4 only");
+                  "Burgers operator variant not built for this degree");
     hip_check(it->second(P, op.d_basis_d.ptr, op.d_burgers.ptr, io,
                          op.stream),
               "burgers launch");
@@ -799,6 +799,92 @@ dg_lsrk_coefficients(int scheme, double *bi, double *ai)
   return (int)b.size();
 }
 
+namespace dgops
+{
+  // one LSRK step: the stages of the plan, fused (the register rotation
+  // ends with the solution back in `solution`)
+  static void
+  lsrk_step(Operator &op, int scheme, double t, double dt, void *const (&buf)[3])
+  {
+    for (const LsrkStage &s : lsrk_plan(scheme))
+      fused_stage(op, t + s.c * dt, s.b * dt, s.a * dt, buf[s.r_in],
+                  buf[s.s_in], buf[s.s_out], s.last ? nullptr : buf[s.r_out],
+                  s.last);
+  }
+
+  // the operator does not depend on time: constant speed factor and no
+  // boundary faces (whose Dirichlet data follow the stage time)
+  static bool
+  autonomous(const Operator &op)
+  {
+    if (op.cfg.time_factor != DG_TIME_FACTOR_ONE)
+      return false;
+    for (const int nb : op.mesh.neighbors)
+      if (nb < 0)
+        return false;
+    return true;
+  }
+
+  // the cached graph of one step for (scheme, dt, registers), captured on
+  // a stream of its own (the caller's stream may be the null stream); the
+  // launches are the same as eager ones, so replays give the same bits
+  static hipGraphExec_t
+  step_graph(Operator &op, int scheme, double dt, void *const (&buf)[3])
+  {
+    auto &g = op.step_graph;
+    if (g.exec && g.scheme == scheme && g.dt == dt && g.reg[0] == buf[0] &&
+        g.reg[1] == buf[1] && g.reg[2] == buf[2])
+      return g.exec;
+    if (g.exec)
+      {
+        (void)hipGraphExecDestroy(g.exec);
+        g.exec = nullptr;
+      }
+    hipStream_t cap;
+    hip_check(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking),
+              "capture stream");
+    hipStream_t saved = op.stream;
+    hipGraph_t  graph = nullptr;
+    op.stream         = cap;
+    try
+      {
+        hip_check(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal),
+                  "begin capture");
+        try
+          {
+            lsrk_step(op, scheme, 0., dt, buf);
+          }
+        catch (...)
+          {
+            hipGraph_t g0 = nullptr;
+            (void)hipStreamEndCapture(cap, &g0);
+            if (g0)
+              (void)hipGraphDestroy(g0);
+            throw;
+          }
+        hip_check(hipStreamEndCapture(cap, &graph), "end capture");
+        hip_check(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0),
+                  "graph instantiate");
+      }
+    catch (...)
+      {
+        op.stream = saved;
+        if (graph)
+          (void)hipGraphDestroy(graph);
+        (void)hipStreamDestroy(cap);
+        throw;
+      }
+    op.stream = saved;
+    (void)hipGraphDestroy(graph);
+    (void)hipStreamDestroy(cap);
+    g.scheme = scheme;
+    g.dt     = dt;
+    for (int i = 0; i < 3; ++i)
+      g.reg[i] = buf[i];
+    return g.exec;
+  }
+} // namespace dgops
+
 int
 dg_lsrk_time_step(dg_operator *h, int scheme, double t, double dt,
                   void *solution, void *vec_ri, void *vec_ki)
@@ -808,11 +894,38 @@ dg_lsrk_time_step(dg_operator *h, int scheme, double t, double dt,
     require(solution && vec_ri && vec_ki, "null vector");
     require(solution != vec_ri && solution != vec_ki && vec_ri != vec_ki,
             "the three registers must be distinct");
-    void *buf[3] = {solution, vec_ri, vec_ki};
-    for (const LsrkStage &s : lsrk_plan(scheme))
-      fused_stage(*op, t + s.c * dt, s.b * dt, s.a * dt, buf[s.r_in],
-                  buf[s.s_in], buf[s.s_out], s.last ? nullptr : buf[s.r_out],
-                  s.last);
+    void *const buf[3] = {solution, vec_ri, vec_ki};
+    lsrk_step(*op, scheme, t, dt, buf);
+  });
+}
+
+int
+dg_lsrk_time_steps(dg_operator *h, int scheme, double t, double dt,
+                   int n_steps, void *solution, void *vec_ri, void *vec_ki)
+{
+  return guard([&] {
+    Operator *op = get(h);
+    require(solution && vec_ri && vec_ki, "null vector");
+    require(solution != vec_ri && solution != vec_ki && vec_ri != vec_ki,
+            "the three registers must be distinct");
+    require(n_steps >= 0, "n_steps must be >= 0");
+    void *const buf[3] = {solution, vec_ri, vec_ki};
+    if (n_steps == 0)
+      return;
+    // the first step eagerly: it builds the tables, tunes the tile shape
+    // and allocates the workspace, none of which a captured step may do
+    lsrk_step(*op, scheme, t, dt, buf);
+    if (n_steps == 1)
+      return;
+    if (!autonomous(*op))
+      {
+        for (int s = 1; s < n_steps; ++s)
+          lsrk_step(*op, scheme, t + s * dt, dt, buf);
+        return;
+      }
+    const hipGraphExec_t exec = step_graph(*op, scheme, dt, buf);
+    for (int s = 1; s < n_steps; ++s)
+      hip_check(hipGraphLaunch(exec, op->stream), "graph launch");
   });
 }
 

@@ -142,6 +142,21 @@ namespace dgops
     TileChoice last_tile;
     uint32_t   last_tile_key = 0;
 
+    // captured HIP graph of one LSRK step (dg_lsrk_time_steps) and what it
+    // was captured for: scheme, time step and the three registers
+    struct StepGraph
+    {
+      hipGraphExec_t exec = nullptr;
+      int            scheme = -1;
+      double         dt     = 0;
+      void          *reg[3] = {nullptr, nullptr, nullptr};
+      ~StepGraph()
+      {
+        if (exec)
+          (void)hipGraphExecDestroy(exec);
+      }
+    } step_graph;
+
     // scratch for reductions and solvers
     DevBuf d_partials, d_result;
     std::vector<DevBuf *> scratch_vectors;

@@ -66,6 +66,14 @@ class LowStorageRungeKuttaIntegrator:
         pde_operator._lsrk_time_step(self.scheme, current_time, time_step,
                                      solution, vec_ri, vec_ki)
 
+    def perform_time_steps(self, pde_operator, current_time, time_step,
+                           n_steps, solution, vec_ri, vec_ki):
+        """n_steps calls of perform_time_step (the driver's time loop,
+        advection_solver.cc:1264-1299) in one library call: for an operator
+        without time dependence the steps replay one captured HIP graph."""
+        pde_operator._lsrk_time_steps(self.scheme, current_time, time_step,
+                                      n_steps, solution, vec_ri, vec_ki)
+
 
 class AdvectionOperation:
     """The reference's AdvectionOperation (explicit: advection_solver.cc:330-421;
@@ -128,6 +136,12 @@ class AdvectionOperation:
     def _lsrk_time_step(self, scheme, t, dt, solution, vec_ri, vec_ki):
         L.check(self._lib.dg_lsrk_time_step(
             self._h, scheme, t, dt, self._v(solution, "solution"),
+            self._v(vec_ri, "vec_ri"), self._v(vec_ki, "vec_ki")))
+
+    def _lsrk_time_steps(self, scheme, t, dt, n_steps, solution, vec_ri,
+                         vec_ki):
+        L.check(self._lib.dg_lsrk_time_steps(
+            self._h, scheme, t, dt, n_steps, self._v(solution, "solution"),
             self._v(vec_ri, "vec_ri"), self._v(vec_ki, "vec_ki")))
 
     def set_tile_shape(self, cells_per_workgroup=0):

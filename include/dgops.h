@@ -201,6 +201,20 @@ int dg_lsrk_time_step(dg_operator *op,
                       void        *solution,
                       void        *vec_ri,
                       void        *vec_ki);
+/* n_steps steps of dg_lsrk_time_step from current_time (the time loop of
+ * AdvectionProblem::run, advection_solver.cc:1264-1299), with the same
+ * result bit for bit. When the operator does not depend on time (constant
+ * speed factor, no boundary faces) the steps after the first replay one
+ * captured HIP graph of a step's launches, cached per (scheme, time step,
+ * registers); otherwise they are launched one by one.                   */
+int dg_lsrk_time_steps(dg_operator *op,
+                       int          scheme,
+                       double       current_time,
+                       double       time_step,
+                       int          n_steps,
+                       void        *solution,
+                       void        *vec_ri,
+                       void        *vec_ki);
 int dg_lsrk_n_stages(int scheme);
 /* the scheme's b_i and a_i (n_stages values each, a_i of the last stage
  * 0), for integrators written on top of dg_perform_stage; returns the

@@ -274,6 +274,49 @@ namespace dgops
     bool         batched_ = false;
   };
 
+  // the reference's LowStorageRungeKuttaIntegrator (advection_solver.cc
+  // :823-944) with the stages fused inside the library: perform_time_step
+  // is one dg_lsrk_time_step; perform_time_steps runs the driver's time
+  // loop (advection_solver.cc:1264-1299) in one call, replaying a captured
+  // HIP graph of a step when the operator does not depend on time
+  class LowStorageRungeKuttaIntegrator
+  {
+  public:
+    explicit LowStorageRungeKuttaIntegrator(const int scheme)
+      : scheme_(scheme)
+    {
+      if (dg_lsrk_n_stages(scheme) < 0)
+        check(DG_ERR_ARGUMENT);
+    }
+
+    unsigned int n_stages() const { return dg_lsrk_n_stages(scheme_); }
+
+    template <typename Op, typename V>
+    void
+    perform_time_step(const Op &op, const double current_time,
+                      const double time_step, V &solution, V &vec_ri,
+                      V &vec_ki) const
+    {
+      check(dg_lsrk_time_step(op.handle(), scheme_, current_time, time_step,
+                              device_data(solution), device_data(vec_ri),
+                              device_data(vec_ki)));
+    }
+
+    template <typename Op, typename V>
+    void
+    perform_time_steps(const Op &op, const double current_time,
+                       const double time_step, const int n_steps,
+                       V &solution, V &vec_ri, V &vec_ki) const
+    {
+      check(dg_lsrk_time_steps(op.handle(), scheme_, current_time, time_step,
+                               n_steps, device_data(solution),
+                               device_data(vec_ri), device_data(vec_ki)));
+    }
+
+  private:
+    int scheme_;
+  };
+
   // BlockJacobi wrapper of the reference (advection_solver_implicit_real.cc
   // :1540-1556): vmult(dst, src) forwards to precondition_block_jacobi
   template <typename Operator>

@@ -3,7 +3,12 @@
 // (perform_stage with the aliasing of advection_solver.cc:906-939) drives
 // the GPU operator through the C++ facade. Writes the solution after
 // `steps` steps of LSRK(5,4) to the file named by argv[1] (raw doubles).
+// argv[3] picks the integrator: "caller" (default, the one below),
+// "library" (dgops::LowStorageRungeKuttaIntegrator, one call per step) or
+// "steps" (its perform_time_steps, all steps in one call).
 #include <cstdio>
+#include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "dgops.hpp"
@@ -54,10 +59,13 @@ main(int argc, char **argv)
 {
   if (argc < 2)
     {
-      std::fprintf(stderr, "usage: %s output.bin [steps]\n", argv[0]);
+      std::fprintf(stderr,
+                   "usage: %s output.bin [steps] [caller|library|steps]\n",
+                   argv[0]);
       return 2;
     }
-  const int steps = argc > 2 ? std::atoi(argv[2]) : 3;
+  const int         steps = argc > 2 ? std::atoi(argv[2]) : 3;
+  const std::string mode  = argc > 3 ? argv[3] : "caller";
   try
     {
       dg_config cfg;
@@ -78,10 +86,26 @@ main(int argc, char **argv)
       op.initialize_dof_vector(ri);
       op.initialize_dof_vector(ki);
       op.project_initial(u, 0.);
-      LowStorageRungeKutta lsrk(DG_LSRK_5_4);
-      const double         dt = 2e-3;
-      for (int s = 0; s < steps; ++s)
-        lsrk.perform_time_step(op, s * dt, dt, u, ri, ki);
+      const double dt = 2e-3;
+      if (mode == "caller")
+        {
+          LowStorageRungeKutta lsrk(DG_LSRK_5_4);
+          for (int s = 0; s < steps; ++s)
+            lsrk.perform_time_step(op, s * dt, dt, u, ri, ki);
+        }
+      else if (mode == "library")
+        {
+          dgops::LowStorageRungeKuttaIntegrator lsrk(DG_LSRK_5_4);
+          for (int s = 0; s < steps; ++s)
+            lsrk.perform_time_step(op, s * dt, dt, u, ri, ki);
+        }
+      else if (mode == "steps")
+        {
+          dgops::LowStorageRungeKuttaIntegrator lsrk(DG_LSRK_5_4);
+          lsrk.perform_time_steps(op, 0., dt, steps, u, ri, ki);
+        }
+      else
+        throw std::runtime_error("unknown mode " + mode);
       std::vector<double> host(u.size());
       if (hipMemcpy(host.data(), u.data(), sizeof(double) * u.size(),
                     hipMemcpyDeviceToHost) != hipSuccess)

@@ -18,15 +18,21 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 DRIVER = os.path.join(HERE, "cpp", "facade_lsrk")
 
 
-def test_facade_lsrk_driver_matches_oracle(tmp_path):
+def _run_lsrk_driver(tmp_path, steps, mode):
     assert os.path.exists(DRIVER), \
         "tests/cpp/facade_lsrk is missing: run __graft_entry__.build()"
-    out = tmp_path / "u.bin"
-    steps = 3
-    r = subprocess.run([DRIVER, str(out), str(steps)], capture_output=True,
-                       text=True, timeout=120)
+    out = tmp_path / f"u_{mode}.bin"
+    print(f"facade_lsrk {mode} {steps} steps", flush=True)
+    r = subprocess.run([DRIVER, str(out), str(steps), mode],
+                       capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    u_gpu = np.fromfile(out, dtype=np.float64)
+    return np.fromfile(out, dtype=np.float64)
+
+
+def test_facade_lsrk_driver_matches_oracle(tmp_path):
+    steps = 3
+    runs = {m: _run_lsrk_driver(tmp_path, steps, m)
+            for m in ("caller", "library", "steps")}
     c = Cfg(dim=3, degree=3, refine=2, perturbation=0.1, velocity_kind=1,
             solution_kind=3, solution_param=[0.5, 0.5, 0.5, 0.15])
     o = c.oracle_handle()
@@ -35,8 +41,11 @@ def test_facade_lsrk_driver_matches_oracle(tmp_path):
     dt = 2e-3
     for s in range(steps):
         o.lsrk_time_step(1, s * dt, dt, u, ri, ki)
-    assert u_gpu.size == u.size
-    assert rel_diff(u_gpu, u) < 1e-12
+    for m, u_gpu in runs.items():
+        assert u_gpu.size == u.size, m
+        assert rel_diff(u_gpu, u) < 1e-12, m
+    # all steps in one call (graph replay) equal the steps one by one
+    assert np.array_equal(runs["steps"], runs["library"])
 
 
 IRK_DRIVER = os.path.join(HERE, "cpp", "facade_irk")
