@@ -18,6 +18,8 @@ DG_VEL_CONSTANT, DG_VEL_CELLULAR_3D, DG_VEL_VORTEX, DG_VEL_VORTEX_PERT = range(4
 (DG_SOL_ZERO, DG_SOL_WAVE, DG_SOL_SINE_PRODUCT, DG_SOL_GAUSSIAN,
  DG_SOL_POLYNOMIAL, DG_SOL_TWO_PULSES, DG_SOL_BURGERS_SINE) = range(7)
 DG_LSRK_3_3, DG_LSRK_5_4, DG_LSRK_7_4, DG_LSRK_9_5 = range(4)
+# steps per captured graph of dg_lsrk_time_steps
+DG_GRAPH_STEPS = 8
 DG_TIME_FACTOR_ONE, DG_TIME_FACTOR_COS = 0, 1
 
 
@@ -112,6 +114,8 @@ SIGNATURES = {
     "dg_perform_stage": (_i, [_vp, _d, _d, _d, _vp, _vp, _vp, _vp]),
     "dg_lsrk_time_step": (_i, [_vp, _i, _d, _d, _vp, _vp, _vp]),
     "dg_lsrk_time_steps": (_i, [_vp, _i, _d, _d, _i, _vp, _vp, _vp]),
+    "dg_set_step_launch": (_i, [_vp, _i]),
+    "dg_get_step_launch": (_i, [_vp, ctypes.POINTER(_i), _dp, _dp]),
     "dg_lsrk_n_stages": (_i, [_i]),
     "dg_lsrk_coefficients": (_i, [_i, _dp, _dp]),
     "dg_project_initial": (_i, [_vp, _vp, _d]),

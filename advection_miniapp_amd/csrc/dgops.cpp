@@ -825,9 +825,10 @@ namespace dgops
     return true;
   }
 
-  // the cached graph of one step for (scheme, dt, registers), captured on
-  // a stream of its own (the caller's stream may be the null stream); the
-  // launches are the same as eager ones, so replays give the same bits
+  // the cached graph of DG_GRAPH_STEPS steps for (scheme, dt, registers),
+  // captured on a stream of its own (the caller's stream may be the null
+  // stream); the launches are the same as eager ones, so replays give the
+  // same bits
   static hipGraphExec_t
   step_graph(Operator &op, int scheme, double dt, void *const (&buf)[3])
   {
@@ -852,7 +853,8 @@ namespace dgops
                   "begin capture");
         try
           {
-            lsrk_step(op, scheme, 0., dt, buf);
+            for (int s = 0; s < DG_GRAPH_STEPS; ++s)
+              lsrk_step(op, scheme, 0., dt, buf);
           }
         catch (...)
           {
@@ -882,6 +884,61 @@ namespace dgops
     for (int i = 0; i < 3; ++i)
       g.reg[i] = buf[i];
     return g.exec;
+  }
+
+  struct EventPair
+  {
+    hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+    EventPair()
+    {
+      for (auto &x : e)
+        hip_check(hipEventCreate(&x), "step timing event");
+    }
+    ~EventPair()
+    {
+      for (auto &x : e)
+        if (x)
+          (void)hipEventDestroy(x);
+    }
+  };
+
+  // steps [s, n) of an autonomous operator: graph replays of
+  // DG_GRAPH_STEPS steps where the launch form is (or is measured to be)
+  // the graph, the rest one by one. The measurement spends
+  // 3 DG_GRAPH_STEPS real steps: one replay to warm the graph, then
+  // DG_GRAPH_STEPS steps in each form between events.
+  static void
+  autonomous_steps(Operator &op, int scheme, double t, double dt, int s,
+                   int n, void *const (&buf)[3])
+  {
+    auto     &g    = op.step_graph;
+    const int mode = g.force ? g.force : g.mode;
+    if (mode == 0 && n - s >= 3 * DG_GRAPH_STEPS)
+      {
+        const hipGraphExec_t exec = step_graph(op, scheme, dt, buf);
+        hip_check(hipGraphLaunch(exec, op.stream), "graph launch");
+        s += DG_GRAPH_STEPS;
+        EventPair ev;
+        hip_check(hipEventRecord(ev.e[0], op.stream), "step timing event");
+        for (int i = 0; i < DG_GRAPH_STEPS; ++i, ++s)
+          lsrk_step(op, scheme, t + s * dt, dt, buf);
+        hip_check(hipEventRecord(ev.e[1], op.stream), "step timing event");
+        hip_check(hipGraphLaunch(exec, op.stream), "graph launch");
+        s += DG_GRAPH_STEPS;
+        hip_check(hipEventRecord(ev.e[2], op.stream), "step timing event");
+        hip_check(hipEventSynchronize(ev.e[2]), "step timing sync");
+        hip_check(hipEventElapsedTime(&g.eager_ms, ev.e[0], ev.e[1]),
+                  "step timing");
+        hip_check(hipEventElapsedTime(&g.graph_ms, ev.e[1], ev.e[2]),
+                  "step timing");
+        g.mode = g.graph_ms < g.eager_ms ? 2 : 1;
+      }
+    if ((g.force ? g.force : g.mode) == 2)
+      for (; n - s >= DG_GRAPH_STEPS; s += DG_GRAPH_STEPS)
+        hip_check(hipGraphLaunch(step_graph(op, scheme, dt, buf), op.stream),
+                  "graph launch");
+    for (; s < n; ++s)
+      lsrk_step(op, scheme, t + s * dt, dt, buf);
   }
 } // namespace dgops
 
@@ -923,9 +980,34 @@ dg_lsrk_time_steps(dg_operator *h, int scheme, double t, double dt,
           lsrk_step(*op, scheme, t + s * dt, dt, buf);
         return;
       }
-    const hipGraphExec_t exec = step_graph(*op, scheme, dt, buf);
-    for (int s = 1; s < n_steps; ++s)
-      hip_check(hipGraphLaunch(exec, op->stream), "graph launch");
+    autonomous_steps(*op, scheme, t, dt, 1, n_steps, buf);
+  });
+}
+
+int
+dg_set_step_launch(dg_operator *h, int mode)
+{
+  return guard([&] {
+    Operator *op = get(h);
+    require(mode >= 0 && mode <= 2,
+            "launch form must be 0 (measure), 1 (stream) or 2 (graph)");
+    op->step_graph.force = mode;
+  });
+}
+
+int
+dg_get_step_launch(const dg_operator *h, int *mode, double *eager_ms,
+                   double *graph_ms)
+{
+  return guard([&] {
+    const Operator *op = get(h);
+    const auto     &g  = op->step_graph;
+    if (mode)
+      *mode = g.force ? g.force : g.mode;
+    if (eager_ms)
+      *eager_ms = g.eager_ms;
+    if (graph_ms)
+      *graph_ms = g.graph_ms;
   });
 }
 

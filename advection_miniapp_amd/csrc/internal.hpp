@@ -142,14 +142,20 @@ namespace dgops
     TileChoice last_tile;
     uint32_t   last_tile_key = 0;
 
-    // captured HIP graph of one LSRK step (dg_lsrk_time_steps) and what it
-    // was captured for: scheme, time step and the three registers
+    // captured HIP graph of a run of LSRK steps (dg_lsrk_time_steps) and
+    // what it was captured for: scheme, time step and the three registers;
+    // `mode` is the launch form the first long call measured faster
+    // (0 not measured, 1 stream launches, 2 graph replay) with both times,
+    // `force` the caller's choice (0 = measure)
     struct StepGraph
     {
       hipGraphExec_t exec = nullptr;
       int            scheme = -1;
       double         dt     = 0;
       void          *reg[3] = {nullptr, nullptr, nullptr};
+      int            force  = 0;
+      int            mode   = 0;
+      float          eager_ms = 0.f, graph_ms = 0.f;
       ~StepGraph()
       {
         if (exec)

@@ -144,6 +144,22 @@ class AdvectionOperation:
             self._h, scheme, t, dt, n_steps, self._v(solution, "solution"),
             self._v(vec_ri, "vec_ri"), self._v(vec_ki, "vec_ki")))
 
+    def set_step_launch(self, mode=0):
+        """Launch form of LowStorageRungeKuttaIntegrator.perform_time_steps:
+        0 measured on the first long call, 1 stream launches, 2 graph
+        replay wherever the operator allows it."""
+        L.check(self._lib.dg_set_step_launch(self._h, mode))
+
+    def step_launch(self):
+        """The launch form in use (0 before it was measured) and the
+        measured time of DG_GRAPH_STEPS steps in each form (ms)."""
+        m = ctypes.c_int()
+        e, g = ctypes.c_double(), ctypes.c_double()
+        L.check(self._lib.dg_get_step_launch(self._h, ctypes.byref(m),
+                                             ctypes.byref(e),
+                                             ctypes.byref(g)))
+        return dict(mode=m.value, eager_ms=e.value, graph_ms=g.value)
+
     def set_tile_shape(self, cells_per_workgroup=0):
         """Force the operator kernels' tile shape; 0 = autotuned."""
         L.check(self._lib.dg_set_tile_shape(self._h, cells_per_workgroup))

@@ -492,7 +492,8 @@ def time_config1_run():
     err, nrm = op.l2_error(u, 2.0)
     seconds = start.elapsed_time(stop) / 1e3
     # the same run with the steps in one call: the operator is autonomous,
-    # so steps 2..n replay one captured HIP graph of a step
+    # so the call times stream launches against replays of a captured HIP
+    # graph of 8 steps on the first 24 steps and keeps the faster form
     ug = op.initialize_dof_vector()
     torch.cuda.synchronize()
     start.record()
@@ -501,8 +502,9 @@ def time_config1_run():
     stop.record()
     torch.cuda.synchronize()
     return dict(steps=n, seconds=seconds, rel_l2_error=err / nrm,
-                seconds_graph_replay=start.elapsed_time(stop) / 1e3,
-                graph_equals_eager=bool(torch.equal(u, ug)))
+                seconds_steps_call=start.elapsed_time(stop) / 1e3,
+                steps_call_launch=op.step_launch(),
+                steps_call_equals_eager=bool(torch.equal(u, ug)))
 
 
 def time_cell_solver(refine=5, reps=20):

@@ -203,10 +203,14 @@ int dg_lsrk_time_step(dg_operator *op,
                       void        *vec_ki);
 /* n_steps steps of dg_lsrk_time_step from current_time (the time loop of
  * AdvectionProblem::run, advection_solver.cc:1264-1299), with the same
- * result bit for bit. When the operator does not depend on time (constant
- * speed factor, no boundary faces) the steps after the first replay one
- * captured HIP graph of a step's launches, cached per (scheme, time step,
- * registers); otherwise they are launched one by one.                   */
+ * result bit for bit. The first step is launched on the stream. When the
+ * operator does not depend on time (constant speed factor, no boundary
+ * faces), runs of DG_GRAPH_STEPS later steps may replay one captured HIP
+ * graph, cached per (scheme, time step, registers): the first call with
+ * at least 3 DG_GRAPH_STEPS further steps times both launch forms on
+ * them and keeps the faster (dg_get_step_launch). Other steps are
+ * launched one by one.                                                   */
+#define DG_GRAPH_STEPS 8
 int dg_lsrk_time_steps(dg_operator *op,
                        int          scheme,
                        double       current_time,
@@ -215,6 +219,13 @@ int dg_lsrk_time_steps(dg_operator *op,
                        void        *solution,
                        void        *vec_ri,
                        void        *vec_ki);
+/* launch form of dg_lsrk_time_steps: 0 measured on the first long call
+ * (default), 1 stream launches only, 2 graph replay wherever allowed     */
+int dg_set_step_launch(dg_operator *op, int mode);
+/* the form in use (1 or 2; 0 before it was measured) and the measured
+ * time of DG_GRAPH_STEPS steps in each form (ms, 0 when not measured)    */
+int dg_get_step_launch(const dg_operator *op, int *mode, double *eager_ms,
+                       double *graph_ms);
 int dg_lsrk_n_stages(int scheme);
 /* the scheme's b_i and a_i (n_stages values each, a_i of the last stage
  * 0), for integrators written on top of dg_perform_stage; returns the
