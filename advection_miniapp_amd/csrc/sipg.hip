@@ -13,8 +13,10 @@ namespace dgops
     launch(const OpParams &P, const DevBasis<double> *B, const double *src,
            double *dst, double scale, int64_t n_cells, hipStream_t stream)
     {
-      hipLaunchKernelGGL((sipg_general_kernel<dim, n>), dim3(n_cells),
-                         dim3(128), 0, stream, P, B, src, dst, scale);
+      using L              = SipgLay<dim, n>;
+      const int64_t blocks = (n_cells + L::CPB - 1) / L::CPB;
+      hipLaunchKernelGGL((sipg_general_kernel<dim, n>), dim3(blocks),
+                         dim3(L::NT), 0, stream, P, B, src, dst, scale);
     }
 
     template <int dim, int n>

@@ -5,10 +5,13 @@
 // (fin_integrate) and the inverse mass from the line kernel, so the apply
 // on such meshes is three launches.
 //
-// One workgroup per cell. The cell's values, its reference gradients at
-// the Gauss points and the integration run as sum-factorized sweeps in LDS;
-// the geometry (J, adj J, det J) is evaluated point by point from the 2^dim
-// corners. Cell term, with the explicit sign convention (sign = -1):
+// CPB cells per workgroup of 256 threads, each cell with its own block of
+// LDS; every phase strides its tasks (a point, face point or node of one of
+// the workgroup's cells) over the threads. The cell's values, its reference
+// gradients at the Gauss points and the integration run as sum-factorized
+// sweeps in LDS; the geometry (J, adj J, det J) is evaluated point by point
+// from the 2^dim corners. Cell term, with the explicit sign convention
+// (sign = -1):
 //   sign * int grad v . nu grad u = sum_q grad_ref v . (sign nu w_q adj(J)
 //   J^-T grad_ref u).
 // Faces are element-centric: each cell evaluates its own and the
@@ -41,32 +44,54 @@ namespace dgops
     {
       return d == 0 ? 1 : d == 1 ? n : n * n;
     }
+    // per-cell LDS block (doubles): own values, the neighbours' values
+    // across the two faces of a direction, six point buffers, the result,
+    // own and neighbours' corners, the face partial sums [face s][own |
+    // neighbour][3 nf] and the face coefficients [face s][4][nf]
+    static constexpr int oU   = 0;
+    static constexpr int oN   = oU + nd;
+    static constexpr int oA   = oN + 2 * nd;
+    static constexpr int oR   = oA + 6 * nd;
+    static constexpr int oX   = oR + nd;
+    static constexpr int oXn  = oX + nv * dim;
+    static constexpr int oFW  = oXn + 2 * nv * dim;
+    static constexpr int oFC  = oFW + 12 * nf;
+    static constexpr int size = oFC + 8 * nf;
+    // threads per workgroup and cells per workgroup: one point task per
+    // thread where the cells allow it, at most 48 KB of LDS
+    static constexpr int NT  = 256;
+    static constexpr int cpb_t = nd >= NT ? 1 : NT / nd;
+    static constexpr int cpb_l = 48 * 1024 / (8 * size);
+    static constexpr int CPB   = cpb_l < 1 ? 1 : cpb_t < cpb_l ? cpb_t : cpb_l;
   };
 
   // out[p] (+)= sum_i M[r][i] in[p | r -> i] along direction dir, r the
   // coordinate of p along dir (trans: M[i][r]); optionally a second term
-  // with (M2, in2)
+  // with (M2, in2). in, in2, out: offsets in the cells' LDS blocks
   template <int dim, int n, int dir, bool trans, bool two>
   __device__ __forceinline__ void
-  sipg_sweep(const double *M, const double *in, const double *M2,
-             const double *in2, double *out)
+  sipg_sweep(double *lds, const double *M, int in, const double *M2, int in2,
+             int out)
   {
-    using L           = SipgLay<dim, n>;
-    constexpr int st  = L::stride(dir);
-    for (int p = threadIdx.x; p < L::nd; p += blockDim.x)
+    using L          = SipgLay<dim, n>;
+    constexpr int st = L::stride(dir);
+    for (int t = threadIdx.x; t < L::CPB * L::nd; t += L::NT)
       {
-        const int r    = (p / st) % n;
-        const int base = p - r * st;
-        double    s    = 0;
+        const int     lc   = t / L::nd, p = t - lc * L::nd;
+        double       *c    = lds + lc * L::size;
+        const int     r    = (p / st) % n;
+        const int     base = p - r * st;
+        double        s    = 0;
 #pragma unroll
         for (int i = 0; i < n; ++i)
           {
-            s = fma(trans ? M[i * n + r] : M[r * n + i], in[base + i * st], s);
+            s = fma(trans ? M[i * n + r] : M[r * n + i], c[in + base + i * st],
+                    s);
             if constexpr (two)
-              s = fma(trans ? M2[i * n + r] : M2[r * n + i], in2[base + i * st],
-                      s);
+              s = fma(trans ? M2[i * n + r] : M2[r * n + i],
+                      c[in2 + base + i * st], s);
           }
-        out[p] = s;
+        c[out + p] = s;
       }
   }
 
@@ -77,84 +102,97 @@ namespace dgops
   };
 
   template <int dim, int n>
-  __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
+  __global__ void __launch_bounds__(256)
   sipg_general_kernel(const OpParams                       P,
                       const DevBasis<double> *__restrict__ Bp,
                       const double *__restrict__           src,
                       double *__restrict__                 dst,
                       double                               scale)
   {
-    using L          = SipgLay<dim, n>;
-    constexpr int nd = L::nd, nf = L::nf, NV = L::nv;
-    __shared__ double su[nd], sn[2][nd], A[6][nd], sres[nd];
-    __shared__ double sX[NV * dim], sXn[2][NV * dim];
-    // per face s of the direction: tangential partial sums [own |
-    // neighbour][3 nf], and the face coefficients (value, d, t1, t2)
-    __shared__ double fw[2][2][3 * nf];
-    __shared__ double fc[2][4][nf];
-    // the 1D tables in LDS (row stride n): S and G at the Gauss points,
-    // and the nodal-basis derivatives at the two faces
+    using L           = SipgLay<dim, n>;
+    constexpr int nd  = L::nd, nf = L::nf, NV = L::nv, NT = L::NT;
+    constexpr int CPB = L::CPB, SZ = L::size;
+    __shared__ double lds[CPB * SZ];
+    __shared__ int    nbr[CPB][2 * dim];
+    // the 1D tables (row stride n): S and G at the Gauss points, and the
+    // nodal-basis derivatives at the two faces
     __shared__ double tS[n * n], tG[n * n], tD[2][n];
 
-    const DevBasis<double> &B    = *Bp;
-    const int64_t           cell = blockIdx.x;
-    const int               tid  = threadIdx.x;
-    const double            nu   = P.viscosity;
-    const double            sg   = P.sign;
+    const DevBasis<double> &B     = *Bp;
+    const int64_t           cell0 = (int64_t)blockIdx.x * CPB;
+    const int               tid   = threadIdx.x;
+    const double            nu    = P.viscosity;
+    const double            sg    = P.sign;
+    auto cell_of = [&](int lc) { return cell0 + lc; };
+    auto active  = [&](int lc) { return cell0 + lc < P.n_cells; };
+    auto A       = [](int k) { return L::oA + k * nd; };
 
-    for (int p = tid; p < nd; p += blockDim.x)
+    for (int t = tid; t < CPB * nd; t += NT)
       {
-        su[p]   = src[cell * nd + p];
-        sres[p] = 0;
+        const int lc = t / nd, p = t - lc * nd;
+        double   *c  = lds + lc * SZ;
+        c[L::oU + p] = active(lc) ? src[cell_of(lc) * nd + p] : 0.;
+        c[L::oR + p] = 0;
       }
-    for (int e = tid; e < NV * dim; e += blockDim.x)
-      sX[e] = P.cell_vertices[cell * NV * dim + e];
-    for (int e = tid; e < n * n; e += blockDim.x)
+    for (int t = tid; t < CPB * NV * dim; t += NT)
+      {
+        const int lc = t / (NV * dim), e = t - lc * NV * dim;
+        lds[lc * SZ + L::oX + e] =
+          active(lc) ? P.cell_vertices[cell_of(lc) * NV * dim + e] : 0.;
+      }
+    for (int t = tid; t < CPB * 2 * dim; t += NT)
+      {
+        const int lc = t / (2 * dim), f = t - lc * 2 * dim;
+        nbr[lc][f]   = active(lc) ? P.neighbors[cell_of(lc) * 2 * dim + f] : -1;
+      }
+    for (int e = tid; e < n * n; e += NT)
       {
         tS[e] = B.S[e / n][e % n];
         tG[e] = B.G[e / n][e % n];
       }
-    for (int e = tid; e < 2 * n; e += blockDim.x)
+    for (int e = tid; e < 2 * n; e += NT)
       tD[e / n][e % n] = e < n ? B.dphi0[e] : B.dphi1[e - n];
     __syncthreads();
 
     // ---- cell term: reference gradients at the Gauss points ----
-    int gidx[3]; // A[] slot of the reference gradient along each direction
+    int gidx[3]; // A() slot of the reference gradient along each direction
     if constexpr (dim == 3)
       {
-        sipg_sweep<3, n, 0, false, false>(tS, su, tS, su, A[0]);
-        sipg_sweep<3, n, 0, false, false>(tG, su, tG, su, A[1]);
+        sipg_sweep<3, n, 0, false, false>(lds, tS, L::oU, tS, L::oU, A(0));
+        sipg_sweep<3, n, 0, false, false>(lds, tG, L::oU, tG, L::oU, A(1));
         __syncthreads();
-        sipg_sweep<3, n, 1, false, false>(tS, A[0], tS, A[0], A[2]);
-        sipg_sweep<3, n, 1, false, false>(tS, A[1], tS, A[1], A[3]);
-        sipg_sweep<3, n, 1, false, false>(tG, A[0], tG, A[0], A[4]);
+        sipg_sweep<3, n, 1, false, false>(lds, tS, A(0), tS, A(0), A(2));
+        sipg_sweep<3, n, 1, false, false>(lds, tS, A(1), tS, A(1), A(3));
+        sipg_sweep<3, n, 1, false, false>(lds, tG, A(0), tG, A(0), A(4));
         __syncthreads();
-        sipg_sweep<3, n, 2, false, false>(tG, A[2], tG, A[2], A[5]);
-        sipg_sweep<3, n, 2, false, false>(tS, A[3], tS, A[3], A[0]);
-        sipg_sweep<3, n, 2, false, false>(tS, A[4], tS, A[4], A[1]);
+        sipg_sweep<3, n, 2, false, false>(lds, tG, A(2), tG, A(2), A(5));
+        sipg_sweep<3, n, 2, false, false>(lds, tS, A(3), tS, A(3), A(0));
+        sipg_sweep<3, n, 2, false, false>(lds, tS, A(4), tS, A(4), A(1));
         gidx[0] = 0;
         gidx[1] = 1;
         gidx[2] = 5;
       }
     else
       {
-        sipg_sweep<2, n, 0, false, false>(tS, su, tS, su, A[0]);
-        sipg_sweep<2, n, 0, false, false>(tG, su, tG, su, A[1]);
+        sipg_sweep<2, n, 0, false, false>(lds, tS, L::oU, tS, L::oU, A(0));
+        sipg_sweep<2, n, 0, false, false>(lds, tG, L::oU, tG, L::oU, A(1));
         __syncthreads();
-        sipg_sweep<2, n, 1, false, false>(tS, A[1], tS, A[1], A[2]);
-        sipg_sweep<2, n, 1, false, false>(tG, A[0], tG, A[0], A[3]);
+        sipg_sweep<2, n, 1, false, false>(lds, tS, A(1), tS, A(1), A(2));
+        sipg_sweep<2, n, 1, false, false>(lds, tG, A(0), tG, A(0), A(3));
         gidx[0] = 2;
         gidx[1] = 3;
         gidx[2] = 3;
       }
     __syncthreads();
     // point-wise: H_d = sign nu w adj(J)_d . J^-T g (in place of g)
-    for (int p = tid; p < nd; p += blockDim.x)
+    for (int t = tid; t < CPB * nd; t += NT)
       {
+        const int lc = t / nd, p = t - lc * nd;
+        double   *c  = lds + lc * SZ;
         const int q[3] = {p % n, (p / n) % n, dim == 3 ? p / (n * n) : 0};
         double    xi[3] = {B.xq[q[0]], B.xq[q[1]], dim == 3 ? B.xq[q[2]] : 0.};
         double    x[3], J[3][3], adj[3][3];
-        multilinear_map<dim>(sX, xi, x, J);
+        multilinear_map<dim>(c + L::oX, xi, x, J);
         const double det = adjugate<dim>(J, adj);
         double       w   = B.w[q[0]] * B.w[q[1]];
         if constexpr (dim == 3)
@@ -162,7 +200,7 @@ namespace dgops
         double g[3] = {0, 0, 0}, gu[3] = {0, 0, 0};
 #pragma unroll
         for (int d = 0; d < dim; ++d)
-          g[d] = A[gidx[d]][p];
+          g[d] = c[A(gidx[d]) + p];
 #pragma unroll
         for (int e = 0; e < dim; ++e)
 #pragma unroll
@@ -176,28 +214,28 @@ namespace dgops
 #pragma unroll
             for (int e = 0; e < dim; ++e)
               h = fma(adj[d][e], gu[e], h);
-            A[gidx[d]][p] = f * h;
+            c[A(gidx[d]) + p] = f * h;
           }
       }
     __syncthreads();
     // integration with the test-function gradients
     if constexpr (dim == 3)
       {
-        sipg_sweep<3, n, 2, true, false>(tS, A[0], tS, A[0], A[2]);
-        sipg_sweep<3, n, 2, true, false>(tS, A[1], tS, A[1], A[3]);
-        sipg_sweep<3, n, 2, true, false>(tG, A[5], tG, A[5], A[4]);
+        sipg_sweep<3, n, 2, true, false>(lds, tS, A(0), tS, A(0), A(2));
+        sipg_sweep<3, n, 2, true, false>(lds, tS, A(1), tS, A(1), A(3));
+        sipg_sweep<3, n, 2, true, false>(lds, tG, A(5), tG, A(5), A(4));
         __syncthreads();
-        sipg_sweep<3, n, 1, true, false>(tS, A[2], tS, A[2], A[0]);
-        sipg_sweep<3, n, 1, true, true>(tG, A[3], tS, A[4], A[1]);
+        sipg_sweep<3, n, 1, true, false>(lds, tS, A(2), tS, A(2), A(0));
+        sipg_sweep<3, n, 1, true, true>(lds, tG, A(3), tS, A(4), A(1));
         __syncthreads();
-        sipg_sweep<3, n, 0, true, true>(tG, A[0], tS, A[1], sres);
+        sipg_sweep<3, n, 0, true, true>(lds, tG, A(0), tS, A(1), L::oR);
       }
     else
       {
-        sipg_sweep<2, n, 1, true, false>(tS, A[2], tS, A[2], A[0]);
-        sipg_sweep<2, n, 1, true, false>(tG, A[3], tG, A[3], A[1]);
+        sipg_sweep<2, n, 1, true, false>(lds, tS, A(2), tS, A(2), A(0));
+        sipg_sweep<2, n, 1, true, false>(lds, tG, A(3), tG, A(3), A(1));
         __syncthreads();
-        sipg_sweep<2, n, 0, true, true>(tG, A[0], tS, A[1], sres);
+        sipg_sweep<2, n, 0, true, true>(lds, tG, A(0), tS, A(1), L::oR);
       }
     __syncthreads();
 
@@ -208,30 +246,35 @@ namespace dgops
       constexpr int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
       constexpr int sd = L::stride(d), s1 = L::stride(t1),
                     s2 = dim == 3 ? L::stride(t2) : 0;
-      // neighbours across the two faces (periodic meshes: both exist)
-      const int nbs[2] = {P.neighbors[cell * 2 * dim + 2 * d],
-                          P.neighbors[cell * 2 * dim + 2 * d + 1]};
-      for (int p = tid; p < 2 * nd; p += blockDim.x)
+      // the neighbours across the two faces (periodic meshes: both exist)
+      for (int t = tid; t < CPB * 2 * nd; t += NT)
         {
-          const int s = p / nd, q = p - s * nd;
-          sn[s][q]    = nbs[s] >= 0 ? src[(int64_t)nbs[s] * nd + q] : 0.;
+          const int lc = t / (2 * nd), e = t - lc * 2 * nd;
+          const int s = e / nd, q = e - s * nd;
+          const int nb = nbr[lc][2 * d + s];
+          lds[lc * SZ + L::oN + e] = nb >= 0 ? src[(int64_t)nb * nd + q] : 0.;
         }
-      for (int e = tid; e < 2 * NV * dim; e += blockDim.x)
+      for (int t = tid; t < CPB * 2 * NV * dim; t += NT)
         {
+          const int lc = t / (2 * NV * dim), e = t - lc * 2 * NV * dim;
           const int s = e / (NV * dim), q = e - s * NV * dim;
-          sXn[s][q]   = nbs[s] >= 0 ?
-                          P.cell_vertices[(int64_t)nbs[s] * NV * dim + q] :
-                          sX[q];
+          const int nb = nbr[lc][2 * d + s];
+          double   *c  = lds + lc * SZ;
+          c[L::oXn + e] = nb >= 0 ?
+                            P.cell_vertices[(int64_t)nb * NV * dim + q] :
+                            c[L::oX + q];
         }
       __syncthreads();
       // layer values and normal derivatives, first tangential direction:
-      // fw[s][side][0..2] = (S v, G v, S dv) at (a, j); side 0 own (layer
+      // FW[s][side][0..2] = (S v, G v, S dv) at (a, j); side 0 own (layer
       // s), side 1 the neighbour across face s (its layer 1 - s)
-      for (int t = tid; t < 4 * nf; t += blockDim.x)
+      for (int t = tid; t < CPB * 4 * nf; t += NT)
         {
-          const int     s = t / (2 * nf), side = (t / nf) % 2, pt = t % nf;
+          const int     lc = t / (4 * nf), e = t - lc * 4 * nf;
+          double       *c  = lds + lc * SZ;
+          const int     s = e / (2 * nf), side = (e / nf) % 2, pt = e % nf;
           const int     a = pt % n, j = dim == 3 ? pt / n : 0;
-          const double *u  = side ? sn[s] : su;
+          const double *u  = side ? c + L::oN + s * nd : c + L::oU;
           const int     ls = side ? 1 - s : s;
           const int     lay = ls ? n - 1 : 0;
           const double *dp = tD[ls];
@@ -249,22 +292,24 @@ namespace dgops
               vg = fma(tG[a * n + i], ui, vg);
               dv = fma(tS[a * n + i], di, dv);
             }
-          double *o = fw[s][side];
+          double *o = c + L::oFW + (2 * s + side) * 3 * nf;
           o[pt]          = v;
           o[nf + pt]     = vg;
           o[2 * nf + pt] = dv;
         }
       __syncthreads();
       // second tangential direction, geometry and flux per face point
-      for (int t = tid; t < 2 * nf; t += blockDim.x)
+      for (int t = tid; t < CPB * 2 * nf; t += NT)
         {
-          const int s = t / nf, pt = t % nf;
+          const int lc = t / (2 * nf), e = t - lc * 2 * nf;
+          double   *c  = lds + lc * SZ;
+          const int s = e / nf, pt = e % nf;
           const int a = pt % n, b = dim == 3 ? pt / n : 0;
           double    val[2], gref[2][3];
 #pragma unroll
           for (int side = 0; side < 2; ++side)
             {
-              const double *o = fw[s][side];
+              const double *o = c + L::oFW + (2 * s + side) * 3 * nf;
               double        v = 0, g1 = 0, g2 = 0, gd = 0;
               if constexpr (dim == 3)
                 {
@@ -296,21 +341,21 @@ namespace dgops
             xi[t2] = B.xq[b];
           double x[3], J[3][3], adj[3][3], Jn[3][3], adjn[3][3];
           xi[d] = s;
-          multilinear_map<dim>(sX, xi, x, J);
+          multilinear_map<dim>(c + L::oX, xi, x, J);
           const double det = adjugate<dim>(J, adj);
           xi[d] = 1 - s;
-          multilinear_map<dim>(sXn[s], xi, x, Jn);
+          multilinear_map<dim>(c + L::oXn + s * NV * dim, xi, x, Jn);
           const double detn = adjugate<dim>(Jn, adjn);
           // area vector of the face (row d of adj J, along +xi_d)
           double len = 0;
 #pragma unroll
-          for (int e = 0; e < dim; ++e)
-            len = fma(adj[d][e], adj[d][e], len);
+          for (int e2 = 0; e2 < dim; ++e2)
+            len = fma(adj[d][e2], adj[d][e2], len);
           len = sqrt(len);
           double nrm[3] = {0, 0, 0};
 #pragma unroll
-          for (int e = 0; e < dim; ++e)
-            nrm[e] = (s ? 1. : -1.) * adj[d][e] / len;
+          for (int e2 = 0; e2 < dim; ++e2)
+            nrm[e2] = (s ? 1. : -1.) * adj[d][e2] / len;
           double JxW = len * B.w[a];
           if constexpr (dim == 3)
             JxW *= B.w[b];
@@ -320,10 +365,10 @@ namespace dgops
           for (int k = 0; k < dim; ++k)
             {
 #pragma unroll
-              for (int e = 0; e < dim; ++e)
+              for (int e2 = 0; e2 < dim; ++e2)
                 {
-                  jn[k]  = fma(adj[k][e], nrm[e], jn[k]);
-                  jnn[k] = fma(adjn[k][e], nrm[e], jnn[k]);
+                  jn[k]  = fma(adj[k][e2], nrm[e2], jn[k]);
+                  jnn[k] = fma(adjn[k][e2], nrm[e2], jnn[k]);
                 }
               jn[k] /= det;
               jnn[k] /= detn;
@@ -340,40 +385,46 @@ namespace dgops
           const double avg     = -0.5 * nu * (dnm + dnp);
           const double viscous = jump * sigma - avg;
           const double cd      = sg * jump * JxW;
-          fc[s][0][pt] = -sg * viscous * JxW;
-          fc[s][1][pt] = cd * jn[d];
-          fc[s][2][pt] = cd * jn[t1];
-          fc[s][3][pt] = dim == 3 ? cd * jn[t2] : 0.;
+          double      *fc      = c + L::oFC + s * 4 * nf;
+          fc[pt]          = -sg * viscous * JxW;
+          fc[nf + pt]     = cd * jn[d];
+          fc[2 * nf + pt] = cd * jn[t1];
+          fc[3 * nf + pt] = dim == 3 ? cd * jn[t2] : 0.;
         }
       __syncthreads();
-      // test functions: along t2 (transposed) into fw[s][0], then along t1
+      // test functions: along t2 (transposed) into FW[s][0], then along t1
       // and onto the nodes; phi_i(s) picks the layer (Gauss-Lobatto nodes),
       // dphi_s carries the normal-derivative coefficient
       if constexpr (dim == 3)
         {
-          for (int t = tid; t < 2 * nf; t += blockDim.x)
+          for (int t = tid; t < CPB * 2 * nf; t += NT)
             {
-              const int s = t / nf, pt = t % nf;
-              const int a = pt % n, i2 = pt / n;
-              double    z0 = 0, z1 = 0, z2 = 0, z3 = 0;
+              const int     lc = t / (2 * nf), e = t - lc * 2 * nf;
+              double       *c  = lds + lc * SZ;
+              const int     s = e / nf, pt = e % nf;
+              const int     a = pt % n, i2 = pt / n;
+              const double *fc = c + L::oFC + s * 4 * nf;
+              double        z0 = 0, z1 = 0, z2 = 0, z3 = 0;
 #pragma unroll
               for (int b = 0; b < n; ++b)
                 {
                   const int o = b * n + a;
-                  z0 = fma(tS[b * n + i2], fc[s][0][o], z0);
-                  z1 = fma(tS[b * n + i2], fc[s][2][o], z1);
-                  z2 = fma(tG[b * n + i2], fc[s][3][o], z2);
-                  z3 = fma(tS[b * n + i2], fc[s][1][o], z3);
+                  z0 = fma(tS[b * n + i2], fc[o], z0);
+                  z1 = fma(tS[b * n + i2], fc[2 * nf + o], z1);
+                  z2 = fma(tG[b * n + i2], fc[3 * nf + o], z2);
+                  z3 = fma(tS[b * n + i2], fc[nf + o], z3);
                 }
-              double *o = fw[s][0];
+              double *o = c + L::oFW + 2 * s * 3 * nf;
               o[pt]          = z0 + z2;
               o[nf + pt]     = z1;
               o[2 * nf + pt] = z3;
             }
           __syncthreads();
         }
-      for (int p = tid; p < nd; p += blockDim.x)
+      for (int t = tid; t < CPB * nd; t += NT)
         {
+          const int lc = t / nd, p = t - lc * nd;
+          double   *c  = lds + lc * SZ;
           const int id = (p / sd) % n, i1 = (p / s1) % n;
           const int i2 = dim == 3 ? (p / s2) % n : 0;
           double    add = 0;
@@ -381,28 +432,29 @@ namespace dgops
           for (int s = 0; s < 2; ++s)
             {
               const double *dp = tD[s];
-              double v0 = 0, v1 = 0;
+              const double *o  = c + L::oFW + 2 * s * 3 * nf;
+              const double *fc = c + L::oFC + s * 4 * nf;
+              double        v0 = 0, v1 = 0;
 #pragma unroll
               for (int a = 0; a < n; ++a)
                 {
                   if constexpr (dim == 3)
                     {
-                      const double *o = fw[s][0];
-                      const int     q = i2 * n + a;
+                      const int q = i2 * n + a;
                       v0 = fma(tS[a * n + i1], o[q],
                                fma(tG[a * n + i1], o[nf + q], v0));
                       v1 = fma(tS[a * n + i1], o[2 * nf + q], v1);
                     }
                   else
                     {
-                      v0 = fma(tS[a * n + i1], fc[s][0][a],
-                               fma(tG[a * n + i1], fc[s][2][a], v0));
-                      v1 = fma(tS[a * n + i1], fc[s][1][a], v1);
+                      v0 = fma(tS[a * n + i1], fc[a],
+                               fma(tG[a * n + i1], fc[2 * nf + a], v0));
+                      v1 = fma(tS[a * n + i1], fc[nf + a], v1);
                     }
                 }
               add += (id == (s ? n - 1 : 0) ? v0 : 0.) + dp[id] * v1;
             }
-          sres[p] += add;
+          c[L::oR + p] += add;
         }
       __syncthreads();
     };
@@ -410,7 +462,11 @@ namespace dgops
     face_pair(FaceDir<1>{});
     if constexpr (dim == 3)
       face_pair(FaceDir<2>{});
-    for (int p = tid; p < nd; p += blockDim.x)
-      dst[cell * nd + p] += scale * sres[p];
+    for (int t = tid; t < CPB * nd; t += NT)
+      {
+        const int lc = t / nd, p = t - lc * nd;
+        if (active(lc))
+          dst[cell_of(lc) * nd + p] += scale * lds[lc * SZ + L::oR + p];
+      }
   }
 } // namespace dgops
