@@ -214,14 +214,16 @@ namespace dgops
 #pragma unroll
           for (int s = 0; s < 2; ++s)
             {
-              double nrm[3] = {0, 0, 0};
+              // normal velocities u.n on both sides with the unit outward
+              // normal n, and the area element len
+              const double *um = tr + 6 * s, *up = um + 3;
+              double        len, unm = 0, unp = 0;
               if constexpr (cartesian)
                 {
-                  double area = 1;
-                  for (int e = 0; e < 3; ++e)
-                    if (e != d)
-                      area *= P.h[e];
-                  nrm[d] = (s ? 1. : -1.) * area * w;
+                  // n = -+e_d, area element h_ta h_tb w
+                  len = P.h[ta] * P.h[tb] * w;
+                  unm = s ? um[d] : -um[d];
+                  unp = s ? up[d] : -up[d];
                 }
               else
                 {
@@ -229,20 +231,21 @@ namespace dgops
                   xi[d]  = s;
                   xi[ta] = sx[qa];
                   xi[tb] = sx[qb];
-                  double x[3], J[3][3], adj[3][3];
+                  double x[3], J[3][3], adj[3][3], nrm[3];
                   multilinear_map<3>(X, xi, x, J);
                   adjugate<3>(J, adj);
                   for (int e = 0; e < 3; ++e)
                     nrm[e] = (s ? 1. : -1.) * adj[d][e] * w;
-                }
-              const double len =
-                sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
-              const double *um = tr + 6 * s, *up = um + 3;
-              double        unm = 0, unp = 0;
-              for (int c = 0; c < 3; ++c)
-                {
-                  unm += um[c] * nrm[c] / len;
-                  unp += up[c] * nrm[c] / len;
+                  len = sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] +
+                             nrm[2] * nrm[2]);
+                  for (int c = 0; c < 3; ++c)
+                    {
+                      unm = fma(um[c], nrm[c], unm);
+                      unp = fma(up[c], nrm[c], unp);
+                    }
+                  const double il = 1. / len;
+                  unm *= il;
+                  unp *= il;
                 }
               const double lam = P.alpha * fmax(fabs(unm), fabs(unp));
               for (int c = 0; c < 3; ++c)
@@ -463,21 +466,20 @@ namespace dgops
       }
 
     // ---- inverse mass with QGauss(k+1): S^-1 diag(1/JxW) S^-T, lines of
-    // the z-line owners (U is free) ----------------------------------------
+    // the z-line owners (U is free); even-odd sweeps, S^-1 being
+    // centrosymmetric ---------------------------------------------------------
     const DevBasis<double> &B = *Bd;
     // S^-T along z: (S^-T v)_q = sum_k Sinv[k][q] v_k
     if (r < n2)
 #pragma unroll
       for (int c = 0; c < 3; ++c)
+        {
+          double v[n];
+          eo_apply<n, false, true, false>(B.eo_Sinv, acc[c], v);
 #pragma unroll
-        for (int q = 0; q < n; ++q)
-          {
-            double v = 0;
-#pragma unroll
-            for (int k = 0; k < n; ++k)
-              v = fma(B.Sinv[k][q], acc[c][k], v);
-            U[c * n3 + q * n2 + r] = v;
-          }
+          for (int q = 0; q < n; ++q)
+            U[c * n3 + q * n2 + r] = v[q];
+        }
     __syncthreads();
     // S^-T along y: thread (i, qz)
     if (r < n2)
@@ -490,15 +492,7 @@ namespace dgops
 #pragma unroll
             for (int j = 0; j < n; ++j)
               v[j] = U[c * n3 + qz * n2 + j * n + i];
-#pragma unroll
-            for (int q = 0; q < n; ++q)
-              {
-                double s = 0;
-#pragma unroll
-                for (int j = 0; j < n; ++j)
-                  s = fma(B.Sinv[j][q], v[j], s);
-                o[q] = s;
-              }
+            eo_apply<n, false, true, false>(B.eo_Sinv, v, o);
 #pragma unroll
             for (int q = 0; q < n; ++q)
               U[c * n3 + qz * n2 + q * n + i] = o[q];
@@ -510,47 +504,41 @@ namespace dgops
       {
         const int qy = r % n, qz = r / n;
         double    ij[n];
-#pragma unroll
-        for (int qx = 0; qx < n; ++qx)
+        if constexpr (cartesian)
           {
-            const double w = B.w[qx] * B.w[qy] * B.w[qz];
-            double       det;
-            if constexpr (cartesian)
-              det = P.h[0] * P.h[1] * P.h[2];
-            else
+            const double iw =
+              B.winv[qy] * B.winv[qz] / (P.h[0] * P.h[1] * P.h[2]);
+#pragma unroll
+            for (int qx = 0; qx < n; ++qx)
+              ij[qx] = B.winv[qx] * iw;
+          }
+        else
+          {
+#pragma unroll
+            for (int qx = 0; qx < n; ++qx)
               {
                 const double xi[3] = {B.xq[qx], B.xq[qy], B.xq[qz]};
                 double       x[3], J[3][3], adj[3][3];
                 multilinear_map<3>(X, xi, x, J);
-                det = adjugate<3>(J, adj);
+                const double det = adjugate<3>(J, adj);
+                ij[qx] = B.winv[qx] * B.winv[qy] * B.winv[qz] / det;
               }
-            ij[qx] = 1. / (det * w);
           }
 #pragma unroll
         for (int c = 0; c < 3; ++c)
           {
-            double v[n], g[n];
+            double v[n], g[n], o[n];
 #pragma unroll
             for (int i = 0; i < n; ++i)
               v[i] = U[c * n3 + qz * n2 + qy * n + i];
+            eo_apply<n, false, true, false>(B.eo_Sinv, v, g);
 #pragma unroll
             for (int q = 0; q < n; ++q)
-              {
-                double s = 0;
-#pragma unroll
-                for (int i = 0; i < n; ++i)
-                  s = fma(B.Sinv[i][q], v[i], s);
-                g[q] = s * ij[q];
-              }
+              g[q] *= ij[q];
+            eo_apply<n, false, false, false>(B.eo_Sinv, g, o);
 #pragma unroll
             for (int i = 0; i < n; ++i)
-              {
-                double s = 0;
-#pragma unroll
-                for (int q = 0; q < n; ++q)
-                  s = fma(B.Sinv[i][q], g[q], s);
-                U[c * n3 + qz * n2 + qy * n + i] = s;
-              }
+              U[c * n3 + qz * n2 + qy * n + i] = o[i];
           }
       }
     __syncthreads();
@@ -565,15 +553,7 @@ namespace dgops
 #pragma unroll
             for (int q = 0; q < n; ++q)
               v[q] = U[c * n3 + qz * n2 + q * n + i];
-#pragma unroll
-            for (int j = 0; j < n; ++j)
-              {
-                double s = 0;
-#pragma unroll
-                for (int q = 0; q < n; ++q)
-                  s = fma(B.Sinv[j][q], v[q], s);
-                o[j] = s;
-              }
+            eo_apply<n, false, false, false>(B.eo_Sinv, v, o);
 #pragma unroll
             for (int j = 0; j < n; ++j)
               U[c * n3 + qz * n2 + j * n + i] = o[j];
@@ -586,26 +566,23 @@ namespace dgops
 #pragma unroll
         for (int c = 0; c < 3; ++c)
           {
-            double v[n];
+            double v[n], o[n];
 #pragma unroll
             for (int q = 0; q < n; ++q)
               v[q] = U[c * n3 + q * n2 + r];
+            eo_apply<n, false, false, false>(B.eo_Sinv, v, o);
 #pragma unroll
             for (int k = 0; k < n; ++k)
               {
-                double s = 0;
-#pragma unroll
-                for (int q = 0; q < n; ++q)
-                  s = fma(B.Sinv[k][q], v[q], s);
-                const int64_t o = cell * nvec + c * n3 + k * n2 + r;
+                const int64_t g = cell * nvec + c * n3 + k * n2 + r;
                 if constexpr (epi == epi_store)
-                  io.dst[o] = io.scale * s;
+                  io.dst[g] = io.scale * o[k];
                 else
                   {
-                    const double s0 = io.sol_in[o];
-                    io.sol_out[o]   = fma(io.b_dt, s, s0);
+                    const double s0 = io.sol_in[g];
+                    io.sol_out[g]   = fma(io.b_dt, o[k], s0);
                     if (!io.last)
-                      io.ri_out[o] = fma(io.a_dt, s, s0);
+                      io.ri_out[g] = fma(io.a_dt, o[k], s0);
                   }
               }
           }
