@@ -22,7 +22,7 @@ namespace dgops
   void aux_inverse_mass(const Operator &op, int p, void *vec);
   // dst += scale * (SIPG diffusion terms of src, integrated form, sign
   // P.sign) on cells of any shape (FP64)
-  void aux_sipg_general(const Operator &op, const OpParams &P,
+  void aux_sipg_general(Operator &op, const OpParams &P,
                         const void *src, void *dst, double scale);
 
   // VTU file of a host solution vector (vtu.cpp)

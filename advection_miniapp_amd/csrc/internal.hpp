@@ -115,6 +115,9 @@ namespace dgops
     // right-hand side in d_gl_k (also for SIPG on perturbed cells)
     bool   gl_mass_only = false;
     DevBuf d_mbasis_d, d_mbasis_f, d_gl_k;
+    // SIPG on perturbed cells: face geometry of the face above every cell
+    // in every direction, built on first use (sipg.hip)
+    DevBuf d_sipg_faces;
 
     // implicit state
     double time      = 0;

@@ -9,8 +9,9 @@
 // LDS; every phase strides its tasks (a point, face point or node of one of
 // the workgroup's cells) over the threads. The cell's values, its reference
 // gradients at the Gauss points and the integration run as sum-factorized
-// sweeps in LDS; the geometry (J, adj J, det J) is evaluated point by point
-// from the 2^dim corners. Cell term, with the explicit sign convention
+// sweeps in LDS; the cell term's geometry (J, adj J, det J) is evaluated
+// point by point from the 2^dim corners, the faces' from a table built once
+// per mesh (sipg_face_table_kernel). Cell term, with the explicit sign convention
 // (sign = -1):
 //   sign * int grad v . nu grad u = sum_q grad_ref v . (sign nu w_q adj(J)
 //   J^-T grad_ref u).
@@ -38,6 +39,9 @@ namespace dgops
     static constexpr int nd = dim == 3 ? n * n * n : n * n;
     static constexpr int nf = dim == 3 ? n * n : n;
     static constexpr int nv = 1 << dim;
+    // face table values per face point: J^-1 n of the cell below the face
+    // (dim), of the cell above it (dim), JxW
+    static constexpr int nt = 2 * dim + 1;
     // stride of the lexicographic node index along direction d
     __device__ static constexpr int
     stride(int d)
@@ -45,16 +49,19 @@ namespace dgops
       return d == 0 ? 1 : d == 1 ? n : n * n;
     }
     // per-cell LDS block (doubles): own values, the neighbours' values
-    // across the two faces of a direction, six point buffers, the result,
-    // own and neighbours' corners, the face partial sums [face s][own |
-    // neighbour][3 nf] and the face coefficients [face s][4][nf]
+    // across the two faces of a direction, six point buffers of the cell
+    // term (in the face passes: the two faces' geometry [face s][nt][nf] as
+    // seen from this cell), the result, the corners, the face partial sums
+    // [face s][own | neighbour][3 nf] and the face coefficients
+    // [face s][4][nf]
     static constexpr int oU   = 0;
     static constexpr int oN   = oU + nd;
     static constexpr int oA   = oN + 2 * nd;
-    static constexpr int oR   = oA + 6 * nd;
+    static constexpr int oG   = oA;
+    static constexpr int nA   = 6 * nd > 2 * nt * nf ? 6 * nd : 2 * nt * nf;
+    static constexpr int oR   = oA + nA;
     static constexpr int oX   = oR + nd;
-    static constexpr int oXn  = oX + nv * dim;
-    static constexpr int oFW  = oXn + 2 * nv * dim;
+    static constexpr int oFW  = oX + nv * dim;
     static constexpr int oFC  = oFW + 12 * nf;
     static constexpr int size = oFC + 8 * nf;
     // threads per workgroup and cells per workgroup: one point task per
@@ -101,10 +108,93 @@ namespace dgops
     static constexpr int value = d;
   };
 
+  // Geometry of the face above each cell in each direction d (its face
+  // xi_d = 1, the neighbour's xi_d = 0), at the face's Gauss points:
+  // T[((cell dim + d) nt + v) nf + pt] = J^-1 n of the cell (v < dim), of
+  // the neighbour (dim <= v < 2 dim), JxW (v = 2 dim), with the cell's unit
+  // outward normal n. Seen from the neighbour the two J^-1 n exchange and
+  // change sign, which gives every cell the face below it. Only interior
+  // vertices are perturbed, so faces across periodic boundaries match.
+  template <int dim, int n, int d>
+  __device__ __forceinline__ void
+  sipg_face_entry(const OpParams &P, const DevBasis<double> &B, int64_t cell,
+                  int pt, double *__restrict__ T)
+  {
+    using L          = SipgLay<dim, n>;
+    constexpr int nf = L::nf, NV = L::nv;
+    constexpr int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+    const int     a = pt % n, b = dim == 3 ? pt / n : 0;
+    double        xi[3] = {0, 0, 0};
+    xi[t1] = B.xq[a];
+    if constexpr (dim == 3)
+      xi[t2] = B.xq[b];
+    double x[3], J[3][3], adj[3][3], Jn[3][3], adjn[3][3];
+    xi[d] = 1;
+    multilinear_map<dim>(P.cell_vertices + cell * NV * dim, xi, x, J);
+    const double det = adjugate<dim>(J, adj);
+    const int    nb  = P.neighbors[cell * 2 * dim + 2 * d + 1];
+    xi[d] = 0;
+    multilinear_map<dim>(P.cell_vertices + (int64_t)(nb >= 0 ? nb : cell) *
+                                             NV * dim,
+                         xi, x, Jn);
+    const double detn = adjugate<dim>(Jn, adjn);
+    // area vector of the face (row d of adj J, along +xi_d)
+    double len = 0;
+#pragma unroll
+    for (int e = 0; e < dim; ++e)
+      len = fma(adj[d][e], adj[d][e], len);
+    len = sqrt(len);
+    double nrm[3] = {0, 0, 0};
+#pragma unroll
+    for (int e = 0; e < dim; ++e)
+      nrm[e] = adj[d][e] / len;
+    double JxW = len * B.w[a];
+    if constexpr (dim == 3)
+      JxW *= B.w[b];
+    double *t = T + ((cell * dim + d) * L::nt) * nf + pt;
+#pragma unroll
+    for (int k = 0; k < dim; ++k)
+      {
+        double jn = 0, jnn = 0;
+#pragma unroll
+        for (int e = 0; e < dim; ++e)
+          {
+            jn  = fma(adj[k][e], nrm[e], jn);
+            jnn = fma(adjn[k][e], nrm[e], jnn);
+          }
+        t[k * nf]         = jn / det;
+        t[(dim + k) * nf] = jnn / detn;
+      }
+    t[2 * dim * nf] = JxW;
+  }
+
+  template <int dim, int n>
+  __global__ void __launch_bounds__(256)
+  sipg_face_table_kernel(const OpParams                       P,
+                         const DevBasis<double> *__restrict__ Bp,
+                         double *__restrict__                 T)
+  {
+    constexpr int nf = SipgLay<dim, n>::nf;
+    const int64_t e  = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= P.n_cells * dim * nf)
+      return;
+    const int     pt   = (int)(e % nf);
+    const int64_t cd   = e / nf;
+    const int     d    = (int)(cd % dim);
+    const int64_t cell = cd / dim;
+    if (d == 0)
+      sipg_face_entry<dim, n, 0>(P, *Bp, cell, pt, T);
+    else if (d == 1)
+      sipg_face_entry<dim, n, 1>(P, *Bp, cell, pt, T);
+    else if constexpr (dim == 3)
+      sipg_face_entry<dim, n, 2>(P, *Bp, cell, pt, T);
+  }
+
   template <int dim, int n>
   __global__ void __launch_bounds__(256)
   sipg_general_kernel(const OpParams                       P,
                       const DevBasis<double> *__restrict__ Bp,
+                      const double *__restrict__           faces,
                       const double *__restrict__           src,
                       double *__restrict__                 dst,
                       double                               scale)
@@ -254,15 +344,22 @@ namespace dgops
           const int nb = nbr[lc][2 * d + s];
           lds[lc * SZ + L::oN + e] = nb >= 0 ? src[(int64_t)nb * nd + q] : 0.;
         }
-      for (int t = tid; t < CPB * 2 * NV * dim; t += NT)
+      // the geometry of the two faces from the table: face 1 is the entry
+      // of this cell, face 0 the entry of the neighbour below, with the two
+      // J^-1 n exchanged and negated (SIPG meshes are periodic: the
+      // neighbour exists)
+      constexpr int nt = L::nt;
+      for (int t = tid; t < CPB * 2 * nt * nf; t += NT)
         {
-          const int lc = t / (2 * NV * dim), e = t - lc * 2 * NV * dim;
-          const int s = e / (NV * dim), q = e - s * NV * dim;
-          const int nb = nbr[lc][2 * d + s];
-          double   *c  = lds + lc * SZ;
-          c[L::oXn + e] = nb >= 0 ?
-                            P.cell_vertices[(int64_t)nb * NV * dim + q] :
-                            c[L::oX + q];
+          const int lc = t / (2 * nt * nf), e = t - lc * 2 * nt * nf;
+          const int s = e / (nt * nf), v = (e / nf) % nt, pt = e % nf;
+          if (!active(lc))
+            continue;
+          const int64_t owner = s ? cell_of(lc) : (int64_t)nbr[lc][2 * d];
+          const int     vs =
+            s || v == 2 * dim ? v : v < dim ? v + dim : v - dim;
+          const double  val = faces[((owner * dim + d) * nt + vs) * nf + pt];
+          lds[lc * SZ + L::oG + e] = s || v == 2 * dim ? val : -val;
         }
       __syncthreads();
       // layer values and normal derivatives, first tangential direction:
@@ -298,7 +395,7 @@ namespace dgops
           o[2 * nf + pt] = dv;
         }
       __syncthreads();
-      // second tangential direction, geometry and flux per face point
+      // second tangential direction and the flux per face point
       for (int t = tid; t < CPB * 2 * nf; t += NT)
         {
           const int lc = t / (2 * nf), e = t - lc * 2 * nf;
@@ -335,44 +432,16 @@ namespace dgops
               if constexpr (dim == 3)
                 gref[side][t2] = g2;
             }
-          double xi[3] = {0, 0, 0};
-          xi[t1] = B.xq[a];
-          if constexpr (dim == 3)
-            xi[t2] = B.xq[b];
-          double x[3], J[3][3], adj[3][3], Jn[3][3], adjn[3][3];
-          xi[d] = s;
-          multilinear_map<dim>(c + L::oX, xi, x, J);
-          const double det = adjugate<dim>(J, adj);
-          xi[d] = 1 - s;
-          multilinear_map<dim>(c + L::oXn + s * NV * dim, xi, x, Jn);
-          const double detn = adjugate<dim>(Jn, adjn);
-          // area vector of the face (row d of adj J, along +xi_d)
-          double len = 0;
-#pragma unroll
-          for (int e2 = 0; e2 < dim; ++e2)
-            len = fma(adj[d][e2], adj[d][e2], len);
-          len = sqrt(len);
-          double nrm[3] = {0, 0, 0};
-#pragma unroll
-          for (int e2 = 0; e2 < dim; ++e2)
-            nrm[e2] = (s ? 1. : -1.) * adj[d][e2] / len;
-          double JxW = len * B.w[a];
-          if constexpr (dim == 3)
-            JxW *= B.w[b];
-          // J^-1 n on both sides and the normal derivatives
-          double jn[3] = {0, 0, 0}, jnn[3] = {0, 0, 0};
+          // J^-1 n with this cell's outward normal on both sides, JxW
+          const double *g = c + L::oG + s * nt * nf + pt;
+          double        jn[3] = {0, 0, 0}, jnn[3] = {0, 0, 0};
 #pragma unroll
           for (int k = 0; k < dim; ++k)
             {
-#pragma unroll
-              for (int e2 = 0; e2 < dim; ++e2)
-                {
-                  jn[k]  = fma(adj[k][e2], nrm[e2], jn[k]);
-                  jnn[k] = fma(adjn[k][e2], nrm[e2], jnn[k]);
-                }
-              jn[k] /= det;
-              jnn[k] /= detn;
+              jn[k]  = g[k * nf];
+              jnn[k] = g[(dim + k) * nf];
             }
+          const double JxW = g[2 * dim * nf];
           double dnm = 0, dnp = 0;
 #pragma unroll
           for (int k = 0; k < dim; ++k)
