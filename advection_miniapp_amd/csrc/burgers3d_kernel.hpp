@@ -137,6 +137,13 @@ namespace dgops
                   "face buffers overlay the cell-term buffers");
     double *FT  = W;
     double *FL  = W + 12 * n * m;
+    // the x and y faces' contributions to their layers, [s][c][b n + a],
+    // in the rest of the overlay (6 n^2 values: what W..F leave after FT and
+    // FL), integrated by all n^2 threads and added by the layer's owners
+    // after the next barrier
+    double *FV = FL + 6 * m2;
+    static_assert(12 * n * m + 6 * m2 + 6 * n2 <= L::oX - L::oW,
+                  "face layer buffer fits the overlay");
     // z-line owners (thread r < n2: i = r % n, j = r / n) accumulate the
     // face and cell terms of their line in registers
     double acc[3][n];
@@ -153,30 +160,59 @@ namespace dgops
     const int     yq = r % m, yy = r / m; // (qa, qb) or (qx, qy)
     const int     gi = r % n, gq = r / n; // (i, qy)
     const int     hi = r % n, hj = r / n; // (i, j) or (a, b)
+    // the owners of the layers of direction dp (0: x, 1: y) add FV
+    auto take_layers = [&](auto dc) {
+      constexpr int dp = decltype(dc)::value;
+      if (t_n2)
+        {
+          const int i = hi, j = hj;
+#pragma unroll
+          for (int s = 0; s < 2; ++s)
+            {
+              const int layer = s ? n - 1 : 0;
+              if (dp == 0 ? i == layer : j == layer)
+                {
+                  const int a = dp == 0 ? j : i;
+#pragma unroll
+                  for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int b = 0; b < n; ++b)
+                      acc[c][b] += FV[(s * 3 + c) * n2 + b * n + a];
+                }
+            }
+        }
+    };
     auto    face_pair = [&](auto dc) {
       constexpr int d = decltype(dc)::value;
       using FI        = FaceIdx<n, 2 * d>; // strides of direction d
       constexpr int ta = FI::ta, tb = FI::tb;
+      if constexpr (d > 0)
+        take_layers(FaceId<d - 1>{});
       // along ta: FT[(s 2 + side) 3 + c][b m + qa], thread (b, qa)
       if (t_nm)
         {
           const int b = xj, qa = xq;
           // row b of the neighbours' layers facing this cell (face s: the
-          // neighbour's layer 1 - s), from global memory
+          // neighbour's layer 1 - s), from global memory; across a boundary
+          // face the row is zero (the load reads the cell's own row, or
+          // cell 0's for a thread without a cell, and is discarded)
           double nv[2][3][n];
 #pragma unroll
           for (int s = 0; s < 2; ++s)
             {
               const int64_t nb =
                 active ? (int64_t)P.neighbors[cell * 6 + 2 * d + s] : -1;
+              const bool    has = nb >= 0;
+              const int64_t nbc = has ? nb : active ? cell : 0;
 #pragma unroll
               for (int c = 0; c < 3; ++c)
 #pragma unroll
                 for (int a = 0; a < n; ++a)
-                  nv[s][c][a] =
-                    nb >= 0 ? io.src[nb * nvec + c * n3 +
-                                     FI::node(a, b, s ? 0 : n - 1)] :
-                              0.;
+                  {
+                    const double v = io.src[nbc * nvec + c * n3 +
+                                            FI::node(a, b, s ? 0 : n - 1)];
+                    nv[s][c][a]    = has ? v : 0.;
+                  }
             }
 #pragma unroll
           for (int k = 0; k < 12; ++k)
@@ -271,10 +307,11 @@ namespace dgops
             }
         }
       __syncthreads();
-      // ... then along ta onto the own layer of side s, by the z-line
-      // owners of its nodes, into their registers: z faces, every owner
-      // (i, j) its node at k = layer; x faces, the owners (layer, j) and
-      // y faces the owners (i, layer) their whole line
+      // ... then along ta onto the own layer of side s: z faces by every
+      // z-line owner (i, j) into its register of node k = layer; x and y
+      // faces by thread (a, b) into FV (node (layer, a, b) of the x faces,
+      // (a, layer, b) of the y faces), added by the owners (layer, j) or
+      // (i, layer) of those lines after the barrier
       if (t_n2)
         {
           const int i = hi, j = hj;
@@ -297,22 +334,17 @@ namespace dgops
                 }
               else
                 {
-                  const bool owner = d == 0 ? i == layer : j == layer;
-                  const int  a     = d == 0 ? j : i;
-                  if (owner)
+                  const int a = i, b = j;
 #pragma unroll
-                    for (int c = 0; c < 3; ++c)
-#pragma unroll
-                      for (int b = 0; b < n; ++b)
-                        {
-                          const double *ft =
-                            FT + (s * 3 + c) * n * m + b * m;
-                          double v = 0;
+                  for (int c = 0; c < 3; ++c)
+                    {
+                      const double *ft = FT + (s * 3 + c) * n * m + b * m;
+                      double        v  = 0;
 #pragma unroll L::UM
-                          for (int qa = 0; qa < m; ++qa)
-                            v = fma(sS[qa * n + a], ft[qa], v);
-                          acc[c][b] += v;
-                        }
+                      for (int qa = 0; qa < m; ++qa)
+                        v = fma(sS[qa * n + a], ft[qa], v);
+                      FV[(s * 3 + c) * n2 + b * n + a] = v;
+                    }
                 }
             }
         }
