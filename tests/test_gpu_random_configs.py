@@ -1,0 +1,134 @@
+"""Seeded random problem configurations: the GPU apply and fused LSRK stage
+against the oracle at 1e-12 relative (FP64). Each draw picks the dimension,
+degree, coarse mesh, refinement, box, periodicity per direction,
+perturbation, speed field, time factor, flux and skew factors, operator
+kind (advection, SIPG on periodic meshes, Burgers) and the quadrature
+switches, within what the library accepts, so that combinations the named
+cases of the other tests do not list are covered too. The draw is printed
+before it runs."""
+import numpy as np
+import pytest
+import torch
+
+from advection_miniapp_amd import AdvectionOperation
+from helpers import Cfg, rel_diff
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+MAX_DOFS = 40000
+
+
+def draw(seed):
+    rng = np.random.default_rng(1000 + seed)
+    dim = int(rng.choice([2, 3]))
+    kind = str(rng.choice(["advection"] * 3 + ["sipg", "burgers"]))
+    kmax = 4 if (kind == "burgers" and dim == 2) else 7
+    k = int(rng.integers(1, kmax + 1))
+    nc = [int(x) for x in rng.integers(1, 4, size=dim)]
+    ncomp = dim if kind == "burgers" else 1
+    nd = (k + 1) ** dim * ncomp
+    if np.prod(nc) * 2 ** dim * nd > MAX_DOFS:
+        nc = [1] * dim
+    refine = 1
+    while (np.prod(nc) * 2 ** (dim * (refine + 1)) * nd <= MAX_DOFS
+           and refine < 4):
+        refine += 1
+    refine = int(rng.integers(1, refine + 1))
+    lo = [float(x) for x in rng.uniform(-1.0, 0.0, size=dim)]
+    hi = [lo[d] + float(rng.uniform(0.5, 2.0)) for d in range(dim)]
+    periodic = [1] * dim
+    if kind == "advection":
+        periodic = [int(x) for x in rng.integers(0, 2, size=dim)]
+    kw = dict(dim=dim, degree=k, n_coarse=nc, refine=refine, lo=lo, hi=hi,
+              periodic=periodic,
+              perturbation=float(rng.choice([0.0, 0.05, 0.15, 0.25])),
+              seed=int(rng.integers(1, 1000)))
+    if kind == "burgers":
+        kw.update(operator_kind=2, n_components=dim, solution_kind=6,
+                  flux_alpha=float(rng.choice([0.5, 1.0])))
+        return kw
+    vk = int(rng.choice([0, 0, 2, 3] + ([1] if dim == 3 else [])))
+    if vk != 0:
+        # the analytic fields are 1-periodic and live on the unit box (the
+        # reference's vortex and cellular problems); on other boxes they are
+        # discontinuous across periodic faces, where the face speed then
+        # depends on the side that evaluates it
+        kw.update(lo=[0.0] * dim, hi=[1.0] * dim)
+    kw.update(velocity_kind=vk,
+              velocity=[float(x) for x in rng.uniform(-1.2, 1.2, size=3)],
+              flux_alpha=float(rng.choice([0.0, 0.5, 1.0])),
+              solution_kind=int(rng.choice([1, 2])),
+              solution_param=[np.pi, 0.0])
+    if vk in (2, 3) and rng.random() < 0.7:
+        kw.update(time_factor=1, final_time=float(rng.uniform(2.0, 8.0)))
+    if kind == "sipg":
+        kw.update(operator_kind=1,
+                  viscosity=float(rng.choice([1e-3, 1e-2, 0.1])),
+                  penalty_factor=float((k + 1) ** 2))
+    else:
+        kw.update(factor_skew=float(rng.choice([0.0, 0.0, 0.5, 1.0])))
+        g = rng.random()
+        if g < 0.15:
+            kw.update(use_gl_quad=1)
+        elif g < 0.3:
+            kw.update(use_gl_quad_mass=1)
+    return kw
+
+
+SEEDS = list(range(32))
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def test_draws_cover_the_axes():
+    """The draws (no GPU work) span the configuration axes they claim."""
+    ds = [draw(s) for s in SEEDS]
+    assert {d["dim"] for d in ds} == {2, 3}
+    assert {d["degree"] for d in ds} >= {1, 2, 3, 4, 5, 6, 7}
+    assert {d.get("operator_kind", 0) for d in ds} == {0, 1, 2}
+    assert any(0 in d["periodic"] for d in ds)
+    assert any(d["perturbation"] > 0 for d in ds)
+    assert any(d.get("time_factor", 0) == 1 for d in ds)
+    assert any(d.get("factor_skew", 0) > 0 for d in ds)
+    assert any(d.get("use_gl_quad", 0) or d.get("use_gl_quad_mass", 0)
+               for d in ds)
+
+
+@pytest.mark.parametrize("seed", SEEDS)
+def test_random_config_apply_and_stage(gpu_lib, seed):
+    kw = draw(seed)
+    print(f"seed {seed}: {kw}", flush=True)
+    c = Cfg(**kw)
+    op, o = AdvectionOperation(c.dg()), c.oracle_handle()
+    rng = np.random.default_rng(seed)
+    n = o.n_dofs * c.n_components
+    u = rng.standard_normal(n)
+    burgers = kw.get("operator_kind", 0) == 2
+    for t in (0.0, 0.37):
+        dst = op.initialize_dof_vector()
+        op.apply(dev(u), dst, t)
+        ref = o.burgers_apply(u) if burgers else o.apply(u, t)
+        assert rel_diff(host(dst), ref) < TOL, (seed, t)
+        if burgers:
+            break
+    # one fused stage with distinct registers (the oracle's stage runs the
+    # scalar operator; the Burgers stage is formed from its right-hand side)
+    ri, sol = rng.standard_normal((2, n))
+    ki_o, sol_o, nri_o = np.zeros(n), sol.copy(), np.zeros(n)
+    if burgers:
+        k_o = o.burgers_apply(ri)
+        sol_o, nri_o = sol + 0.013 * k_o, sol + 0.029 * k_o
+    else:
+        o.perform_stage(0.21, 0.013, 0.029, ri, ki_o, sol_o, nri_o)
+    d_sol, d_ki, d_nri = dev(sol), op.initialize_dof_vector(), \
+        op.initialize_dof_vector()
+    op.perform_stage(0.21, 0.013, 0.029, dev(ri), d_ki, d_sol, d_nri)
+    assert rel_diff(host(d_sol), sol_o) < TOL, seed
+    assert rel_diff(host(d_nri), nri_o) < TOL, seed
