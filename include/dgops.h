@@ -220,7 +220,10 @@ int dg_lsrk_time_steps(dg_operator *op,
                        void        *vec_ri,
                        void        *vec_ki);
 /* launch form of dg_lsrk_time_steps: 0 measured on the first long call
- * (default), 1 stream launches only, 2 graph replay wherever allowed     */
+ * (default), 1 stream launches only, 2 graph replay wherever allowed. No
+ * counterpart in the reference (its time loop calls perform_time_step
+ * per step, advection_solver.cc:1264-1299); the form changes the time,
+ * never the result.                                                      */
 int dg_set_step_launch(dg_operator *op, int mode);
 /* the form in use (1 or 2; 0 before it was measured) and the measured
  * time of DG_GRAPH_STEPS steps in each form (ms, 0 when not measured)    */
