@@ -507,6 +507,62 @@ def test_burgers_conserves_momentum():
         assert abs(m.sum()) < 1e-11 * np.abs(m).sum()
 
 
+# LSRK(5,4) of Kennedy, Carpenter and Lewis (2000), the rationals of
+# advection_solver.cc:840-848
+LSRK54_B = [1153189308089. / 22510343858157., 1772645290293. / 4653164025191.,
+            -1672844663538. / 4480602732383., 2114624349019. / 3568978502595.,
+            5198255086312. / 14908931495163.]
+LSRK54_A = [970286171893. / 4311952581923., 6584761158862. / 12103376702013.,
+            2251764453980. / 15575788980749., 26877169314380. / 34165994151039.]
+
+
+def _burgers_exact(x, t):
+    """u = -sin(pi (x - 2 u t)), the solution of u_t + (u u)_x = 0 (the
+    first component of the flux u (x) u, burgers_solver.cc:277-280) from
+    u0 = -sin(pi x) along the characteristics of speed 2u, by Newton per
+    point (t below the break time 1/(2 pi))."""
+    u = -np.sin(np.pi * x)
+    for _ in range(60):
+        a = np.pi * (x - 2.0 * u * t)
+        u = u - (u + np.sin(a)) / (1.0 - 2.0 * np.pi * t * np.cos(a))
+    return u
+
+
+@pytest.mark.parametrize("k", [2, 3])
+def test_burgers_converges_to_characteristic_solution(k):
+    """2D Burgers from u0 = (-sin(pi x), 0) on [-1,1]^2, periodic, to
+    t = 0.05 (a third of the way to the shock at 1/(2 pi)): the L2 error
+    against the exact solution by characteristics falls at order k+1 under
+    refinement (measured: k=2 2.7, 3.1; k=3 4.1, 3.9), and the second
+    component stays zero. The time loop restates the two-register recursion
+    of advection_solver.cc:906-939 over the oracle's right-hand side."""
+    T, errs = 0.05, []
+    for r in (3, 4, 5):
+        o = _burgers(dim=2, k=k, refine=r).oracle_handle()
+        u = o.project(0.0)
+        h = 2.0 / 2 ** r
+        n = int(np.ceil(T / (0.25 * h / (k + 1) ** 1.5)))
+        dt = T / n
+        for _ in range(n):
+            ri = u
+            for i, b in enumerate(LSRK54_B):
+                kv = o.burgers_apply(ri)
+                if i < len(LSRK54_A):
+                    ri = u + LSRK54_A[i] * dt * kv
+                u = u + b * dt * kv
+        nd = (k + 1) ** 2
+        uc = u.reshape(-1, 2, nd)
+        pts = nodal_points(o.cell_vertices(), k + 1, 2)
+        # L2 error of the first component against the exact solution's
+        # nodal interpolant, with the oracle's (pinned) mass matrix
+        e = np.ascontiguousarray(uc[:, 0, :] - _burgers_exact(pts[..., 0],
+                                                              T)).ravel()
+        errs.append(np.sqrt(e @ o.mass(e)))
+        assert np.max(np.abs(uc[:, 1, :])) < 1e-13
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert errs[-1] < 2e-4 and min(rates) > k + 0.5, (errs, rates)
+
+
 # ---- implicit and IRK operators: compositions of pinned pieces ------------
 
 IMPLICIT = dict(dim=2, degree=3, refine=2, perturbation=0.15,
