@@ -1,5 +1,5 @@
-"""Seeded random problem configurations: the GPU apply and fused LSRK stage
-against the oracle at 1e-12 relative (FP64). Each draw picks the dimension,
+"""Seeded random problem configurations: the GPU apply, fused LSRK stage and
+implicit operator against the oracle at 1e-12 relative (FP64). Each draw picks the dimension,
 degree, coarse mesh, refinement, box, periodicity per direction,
 perturbation, speed field, time factor, flux and skew factors, operator
 kind (advection, SIPG on periodic meshes, Burgers) and the quadrature
@@ -132,3 +132,28 @@ def test_random_config_apply_and_stage(gpu_lib, seed):
     op.perform_stage(0.21, 0.013, 0.029, dev(ri), d_ki, d_sol, d_nri)
     assert rel_diff(host(d_sol), sol_o) < TOL, seed
     assert rel_diff(host(d_nri), nri_o) < TOL, seed
+
+
+def implicit_draw(kw):
+    """The implicit operator is built for advection and SIPG with Gauss
+    quadrature (not for Burgers or the quadrature switches)."""
+    return (kw.get("operator_kind", 0) != 2 and not kw.get("use_gl_quad", 0)
+            and not kw.get("use_gl_quad_mass", 0))
+
+
+VMULT_SEEDS = [s for s in SEEDS if implicit_draw(draw(s))]
+
+
+@pytest.mark.parametrize("seed", VMULT_SEEDS)
+def test_random_config_implicit_vmult(gpu_lib, seed):
+    """M/dt + L(t + dt) (and L alone for dt = 0) against the oracle."""
+    kw = draw(seed)
+    print(f"seed {seed}: {kw}", flush=True)
+    c = Cfg(**kw)
+    op, o = AdvectionOperation(c.dg()), c.oracle_handle()
+    u = np.random.default_rng(50 + seed).standard_normal(o.n_dofs)
+    for t, dt in ((0.1, 0.02), (0.3, 0.0)):
+        op.set_time(t, dt)
+        dst = op.initialize_dof_vector()
+        op.vmult(dst, dev(u))
+        assert rel_diff(host(dst), o.vmult(u, t, dt)) < TOL, (seed, t, dt)
