@@ -349,17 +349,23 @@ namespace dgops
       // J^-1 n exchanged and negated (SIPG meshes are periodic: the
       // neighbour exists)
       constexpr int nt = L::nt;
-      for (int t = tid; t < CPB * 2 * nt * nf; t += NT)
+      for (int t = tid; t < CPB * 2 * nf; t += NT)
         {
-          const int lc = t / (2 * nt * nf), e = t - lc * 2 * nt * nf;
-          const int s = e / (nt * nf), v = (e / nf) % nt, pt = e % nf;
+          const int lc = t / (2 * nf), e = t - lc * 2 * nf;
+          const int s = e / nf, pt = e - s * nf;
           if (!active(lc))
             continue;
           const int64_t owner = s ? cell_of(lc) : (int64_t)nbr[lc][2 * d];
-          const int     vs =
-            s || v == 2 * dim ? v : v < dim ? v + dim : v - dim;
-          const double  val = faces[((owner * dim + d) * nt + vs) * nf + pt];
-          lds[lc * SZ + L::oG + e] = s || v == 2 * dim ? val : -val;
+          const double *f     = faces + (owner * dim + d) * nt * nf + pt;
+          double       *g     = lds + lc * SZ + L::oG + s * nt * nf + pt;
+#pragma unroll
+          for (int k = 0; k < dim; ++k)
+            {
+              const double own = f[k * nf], other = f[(dim + k) * nf];
+              g[k * nf]         = s ? own : -other;
+              g[(dim + k) * nf] = s ? other : -own;
+            }
+          g[2 * dim * nf] = f[2 * dim * nf];
         }
       __syncthreads();
       // layer values and normal derivatives, first tangential direction:
