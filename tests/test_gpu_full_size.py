@@ -1,7 +1,8 @@
 """Properties at the full sizes of the benchmark configurations (too large
 for the oracle): configuration 2 (3D, degree 4, 64^3 perturbed cells,
 cellular speed, 32,768,000 DoF) and configuration 4 (3D, degree 4, 32^3
-cells, Radau IIA(3), 12,288,000 unknowns)."""
+cells, Radau IIA(3), 12,288,000 unknowns) with its Burgers add-on (degree 3,
+48^3 cells, 21,233,664 unknowns)."""
 import numpy as np
 import pytest
 import torch
@@ -107,3 +108,48 @@ def test_config4_irk_solve_reaches_tolerance_at_full_size(gpu_lib):
         op.vmult_irk(res, z)
         res -= rhs
         assert (res.norm() / rhs.norm()).item() < 1e-10, kw
+
+
+def burgers_exact(x, t):
+    """u = -sin(pi (x - 2 u t)): the first component of the Burgers flux
+    u (x) u moves at speed 2u; Newton per point below the break time
+    1/(2 pi)."""
+    u = -torch.sin(np.pi * x)
+    for _ in range(60):
+        a = np.pi * (x - 2.0 * u * t)
+        u = u - (u + torch.sin(a)) / (1.0 - 2.0 * np.pi * t * torch.cos(a))
+    return u
+
+
+def test_config4_burgers_converges_to_characteristic_solution(gpu_lib):
+    """The Burgers add-on at its own size (degree 3, 48^3 cells, 21.2 M
+    unknowns) and at 24^3: LSRK(5,4) runs from u0 = (-sin pi x, 0, 0) to
+    t = 0.05 against the exact solution by characteristics; the error
+    falls at order > 3 from 24^3 to 48^3 and the other two components stay
+    zero."""
+    from helpers import nodal_points
+    T, k, errs = 0.05, 3, []
+    for refine in (3, 4):
+        c = Cfg(dim=3, degree=k, n_components=3, operator_kind=2,
+                n_coarse=[3, 3, 3], refine=refine, lo=[-1, -1, -1],
+                hi=[1, 1, 1], solution_kind=6)
+        op = AdvectionOperation(c.dg())
+        print(f"burgers {3 * 2 ** refine}^3: {op.vector_size} unknowns",
+              flush=True)
+        u = op.initialize_dof_vector()
+        op.project_initial(u, 0.0)
+        r1, r2 = op.initialize_dof_vector(), op.initialize_dof_vector()
+        h = 2.0 / (3 * 2 ** refine)
+        n = int(np.ceil(T / (0.25 * h / (k + 1) ** 1.5)))
+        integ = LowStorageRungeKuttaIntegrator(L.DG_LSRK_5_4)
+        integ.perform_time_steps(op, 0.0, T / n, n, u, r1, r2)
+        uc = u.view(-1, 3, (k + 1) ** 3)
+        x = torch.from_numpy(np.ascontiguousarray(
+            nodal_points(op.cell_vertices(), k + 1, 3)[..., 0])).cuda()
+        errs.append((uc[:, 0, :] - burgers_exact(x, T)).abs().max().item())
+        assert uc[:, 1:, :].abs().max().item() == 0.0
+        del op, u, r1, r2, uc, x
+        torch.cuda.empty_cache()
+    rate = np.log2(errs[0] / errs[1])
+    print(f"max nodal errors {errs}, rate {rate:.2f}", flush=True)
+    assert errs[1] < 1e-4 and rate > 3.0, (errs, rate)
