@@ -2,7 +2,8 @@
 for the oracle): configuration 2 (3D, degree 4, 64^3 perturbed cells,
 cellular speed, 32,768,000 DoF) and configuration 4 (3D, degree 4, 32^3
 cells, Radau IIA(3), 12,288,000 unknowns) with its Burgers add-on (degree 3,
-48^3 cells, 21,233,664 unknowns)."""
+48^3 cells, 21,233,664 unknowns), and configuration 3 (SIPG, 32^3 cells,
+degrees 3, 5, 7)."""
 import numpy as np
 import pytest
 import torch
@@ -151,5 +152,37 @@ def test_config4_burgers_converges_to_characteristic_solution(gpu_lib):
         del op, u, r1, r2, uc, x
         torch.cuda.empty_cache()
     rate = np.log2(errs[0] / errs[1])
+    # measured: 4.0e-5 and 3.3e-6, rate 3.59
     print(f"max nodal errors {errs}, rate {rate:.2f}", flush=True)
     assert errs[1] < 1e-4 and rate > 3.0, (errs, rate)
+
+
+def test_config3_sipg_runs_match_the_decaying_exact_solution(gpu_lib):
+    """Configuration 3 at its own size (32^3 cells, degrees 3, 5, 7, up to
+    16.8 M DoF): LSRK(5,4) runs of advection + SIPG from the product of
+    sines to t = 0.01 against the exact solution, which is the same
+    product translated by a t and damped by exp(-3 nu (2 pi)^2 t); the
+    relative L2 error falls with the degree."""
+    import math
+    T, errs = 0.01, []
+    for k in (3, 5, 7):
+        c = Cfg(dim=3, degree=k, refine=5, velocity=[1.0, 0.5, 0.25],
+                operator_kind=1, viscosity=1e-3,
+                penalty_factor=(k + 1) ** 2, solution_kind=2,
+                solution_param=[2 * math.pi, 1e-3])
+        op = AdvectionOperation(c.dg())
+        print(f"sipg k={k}: {op.n_dofs} DoF", flush=True)
+        u = op.initialize_dof_vector()
+        op.project_initial(u, 0.0)
+        r1, r2 = op.initialize_dof_vector(), op.initialize_dof_vector()
+        n = int(math.ceil(T / (0.1 / 32 / (k + 1) ** 2)))
+        LowStorageRungeKuttaIntegrator(L.DG_LSRK_5_4).perform_time_steps(
+            op, 0.0, T / n, n, u, r1, r2)
+        err, nrm = op.l2_error(u, T)
+        errs.append(err / nrm)
+        del op, u, r1, r2
+        torch.cuda.empty_cache()
+    # measured: 7.7e-7, 7.4e-11, 3.3e-15
+    print(f"relative L2 errors {errs}", flush=True)
+    assert errs[0] > errs[1] > errs[2], errs
+    assert errs[0] < 5e-6 and errs[1] < 1e-9 and errs[2] < 1e-12, errs
