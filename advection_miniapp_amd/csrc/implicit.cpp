@@ -118,6 +118,10 @@ namespace dgops
     setup_irk(Operator &op)
     {
       require_gauss_quadrature(op);
+      require(op.cfg.operator_kind == DG_OP_ADVECTION,
+              "Radau IIA operators: built for advection (the IRK solvers of "
+              "advection_solver_irk_real.cc solve advection only)",
+              DG_ERR_UNSUPPORTED);
       const int s = op.cfg.irk_stages;
       require(s >= 1 && s <= 4, "IRK needs 1..4 stages", DG_ERR_UNSUPPORTED);
       if (op.irk.s == s)
@@ -431,9 +435,6 @@ namespace dgops
               void *dst)
     {
       setup_irk(op);
-      require(!op.sipg_general(),
-              "IRK operators with SIPG need axis-aligned cells",
-              DG_ERR_UNSUPPORTED);
       const int s = op.irk.s;
       OpParams  P = op.base_params(t, s);
       P.sign      = 1.;
@@ -480,9 +481,6 @@ namespace dgops
                   const void *src, void *dst)
     {
       setup_irk(op);
-      require(!op.sipg_general(),
-              "IRK operators with SIPG need axis-aligned cells",
-              DG_ERR_UNSUPPORTED);
       const int s = op.irk.s;
       OpParams  P = op.base_params(t, s);
       P.sign      = 1.;

@@ -157,3 +157,41 @@ def test_random_config_implicit_vmult(gpu_lib, seed):
         dst = op.initialize_dof_vector()
         op.vmult(dst, dev(u))
         assert rel_diff(host(dst), o.vmult(u, t, dt)) < TOL, (seed, t, dt)
+
+
+def irk_draw(kw):
+    """The Radau IIA operators: advection (as the reference's IRK solvers),
+    Gauss quadrature, up to degree 6."""
+    return (implicit_draw(kw) and kw["degree"] <= 6
+            and kw.get("operator_kind", 0) == 0)
+
+
+IRK_SEEDS = [s for s in SEEDS if irk_draw(draw(s))]
+
+
+@pytest.mark.parametrize("seed", IRK_SEEDS)
+def test_random_config_irk_operators(gpu_lib, seed):
+    """The stage-coupled operator, its right-hand side and the solution
+    update of Radau IIA with 2 or 3 stages (the 4-stage table of the
+    reference holds 7-digit literals) against the oracle."""
+    kw = dict(draw(seed), irk_stages=2 + seed % 2)
+    print(f"seed {seed}: {kw}", flush=True)
+    c = Cfg(**kw)
+    s = kw["irk_stages"]
+    op, o = AdvectionOperation(c.dg()), c.oracle_handle()
+    rng = np.random.default_rng(80 + seed)
+    u = rng.standard_normal(o.n_dofs)
+    Z = rng.standard_normal(o.n_dofs * s)
+    t, dt = 0.2, 0.05
+    op.set_time(t, dt)
+    rhs = op.initialize_dof_vector(blocks=s)
+    op.compute_rhs(rhs, dev(u))
+    assert rel_diff(host(rhs), o.irk_compute_rhs(u, t, dt)) < TOL, seed
+    dst = op.initialize_dof_vector(blocks=s)
+    op.vmult_irk(dst, dev(Z))
+    assert rel_diff(host(dst), o.irk_vmult(Z, t, dt)) < TOL, seed
+    sol = dev(u)
+    op.update_solution(sol, dev(Z))
+    u_o = u.copy()
+    o.irk_update_solution(u_o, Z, dt)
+    assert rel_diff(host(sol), u_o) < TOL, seed

@@ -188,3 +188,16 @@ def test_sipg_perturbed_fp32_and_irk_are_refused(gpu_lib):
     with pytest.raises(L.DgError) as e:
         op.apply(op.initialize_dof_vector(), dst, 0.0)
     assert e.value.code == L.DG_ERR_UNSUPPORTED
+    # the Radau IIA operators are built for advection, as the reference's
+    # IRK solvers (on perturbed and on axis-aligned cells)
+    for cfg in (sipg_perturbed("3d_k3_const"), sipg(3, 3)):
+        op = AdvectionOperation(cfg.dg())
+        op.set_time(0.0, 0.01)
+        s = cfg.irk_stages
+        z = op.initialize_dof_vector(blocks=s)
+        with pytest.raises(L.DgError) as e:
+            op.vmult_irk(op.initialize_dof_vector(blocks=s), z)
+        assert e.value.code == L.DG_ERR_UNSUPPORTED
+        with pytest.raises(L.DgError) as e:
+            op.compute_rhs(z, op.initialize_dof_vector())
+        assert e.value.code == L.DG_ERR_UNSUPPORTED
