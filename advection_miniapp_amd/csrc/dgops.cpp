@@ -412,6 +412,12 @@ namespace dgops
                   "advection-diffusion: periodic meshes only (the reference "
                   "has no boundary faces for SIPG)",
                   DG_ERR_UNSUPPORTED);
+      if (c.operator_kind == DG_OP_BURGERS)
+        for (int d = 0; d < c.dim; ++d)
+          require(c.periodic[d],
+                  "Burgers: periodic meshes only (the reference's boundary "
+                  "face term is not implemented, burgers_solver.cc:347-355)",
+                  DG_ERR_UNSUPPORTED);
     }
   } // namespace
 

@@ -42,6 +42,16 @@ def test_config_default_and_validation_without_gpu():
     assert lib.dg_create(ctypes.byref(cfg), None, ctypes.byref(h)) == \
         _lib.DG_ERR_UNSUPPORTED
     assert b"too large" in lib.dg_last_error()
+    # SIPG and Burgers are periodic-only, as in the reference
+    cfg.refine = 2
+    for kind, ncomp in ((_lib.DG_OP_ADVECTION_DIFFUSION, 1),
+                        (_lib.DG_OP_BURGERS, 3)):
+        cfg.operator_kind, cfg.n_components = kind, ncomp
+        cfg.periodic[1] = 0
+        assert lib.dg_create(ctypes.byref(cfg), None, ctypes.byref(h)) == \
+            _lib.DG_ERR_UNSUPPORTED
+        assert b"periodic meshes only" in lib.dg_last_error()
+        cfg.periodic[1] = 1
     assert lib.dg_lsrk_n_stages(_lib.DG_LSRK_5_4) == 5
     assert lib.dg_lsrk_n_stages(_lib.DG_LSRK_9_5) == 9
     assert lib.dg_lsrk_n_stages(17) == -1
