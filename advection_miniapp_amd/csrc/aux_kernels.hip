@@ -22,6 +22,9 @@ namespace dgops
       int    n2;             // points of the second rule
       double S2[9][max_n];   // nodal basis at the n2-point Gauss rule
       double w2[9], x2[9];
+      // the n-point Gauss rule, also where the operator integrates with
+      // Gauss-Lobatto points (use_gl_quad)
+      double w1[max_n], x1[max_n];
     };
 
     // tensor-product sweep on LDS data: out has extent m along `dir`,
@@ -192,8 +195,8 @@ namespace dgops
     }
 
     // per cell: squared L2 error with the (n+1)-point rule and squared
-    // norm of the analytic solution with the n-point rule
-    // (advection_solver.cc:1187-1208)
+    // norm of the analytic solution with the n-point Gauss rule
+    // (advection_solver.cc:1187-1208), whatever the operator's quadrature
     template <int dim>
     __global__ void
     l2_error_kernel(const OpParams P, const DevBasis<double> *__restrict__ B,
@@ -244,11 +247,11 @@ namespace dgops
       for (int pt = threadIdx.x; pt < nd; pt += blockDim.x)
         {
           const int q[3] = {pt % n, (pt / n) % n, dim == 3 ? pt / (n * n) : 0};
-          double    xi[3] = {B->xq[q[0]], B->xq[q[1]], dim == 3 ? B->xq[q[2]] : 0.};
+          double    xi[3] = {Q.x1[q[0]], Q.x1[q[1]], dim == 3 ? Q.x1[q[2]] : 0.};
           double    x[3], J[3][3], adj[3][3];
           multilinear_map<dim>(X, xi, x, J);
           const double det = adjugate<dim>(J, adj);
-          const double w = B->w[q[0]] * B->w[q[1]] * (dim == 3 ? B->w[q[2]] : 1.);
+          const double w = Q.w1[q[0]] * Q.w1[q[1]] * (dim == 3 ? Q.w1[q[2]] : 1.);
           const double g = analytic_solution<dim>(P.sol_kind, P.sol_param, t,
                                                   P.vel_const, x);
           nrm += g * g * det * w;
@@ -877,6 +880,12 @@ namespace dgops
       QuadTables Q = {};
       Q.n2         = n + 1;
       std::vector<double> gp, gw;
+      gauss_points(n, gp, gw);
+      for (int q = 0; q < n; ++q)
+        {
+          Q.w1[q] = gw[q];
+          Q.x1[q] = gp[q];
+        }
       gauss_points(n + 1, gp, gw);
       const auto xl = gauss_lobatto_points_ext(n);
       for (int q = 0; q < n + 1; ++q)

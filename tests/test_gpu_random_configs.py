@@ -1,5 +1,6 @@
-"""Seeded random problem configurations: the GPU apply, fused LSRK stage and
-implicit operator against the oracle at 1e-12 relative (FP64). Each draw picks the dimension,
+"""Seeded random problem configurations: the GPU apply, fused LSRK stage,
+implicit and Radau IIA operators and the auxiliary integrals against the
+oracle at 1e-12 relative (FP64). Each draw picks the dimension,
 degree, coarse mesh, refinement, box, periodicity per direction,
 perturbation, speed field, time factor, flux and skew factors, operator
 kind (advection, SIPG on periodic meshes, Burgers) and the quadrature
@@ -195,3 +196,31 @@ def test_random_config_irk_operators(gpu_lib, seed):
     u_o = u.copy()
     o.irk_update_solution(u_o, Z, dt)
     assert rel_diff(host(sol), u_o) < TOL, seed
+
+
+AUX_SEEDS = [s for s in SEEDS if draw(s).get("operator_kind", 0) != 2]
+
+
+@pytest.mark.parametrize("seed", AUX_SEEDS)
+def test_random_config_auxiliary(gpu_lib, seed):
+    """Projection of the analytic solution (at two times), mass and
+    energy, the L2 error and the standalone inverse mass (scalar fields)
+    against the oracle."""
+    kw = draw(seed)
+    print(f"seed {seed}: {kw}", flush=True)
+    c = Cfg(**kw)
+    op, o = AdvectionOperation(c.dg()), c.oracle_handle()
+    for t in (0.0, 0.37):
+        ref = o.project(t)
+        u = op.initialize_dof_vector()
+        op.project_initial(u, t)
+        assert rel_diff(host(u), ref) < TOL, (seed, t)
+    assert np.allclose(op.compute_mass_and_energy(u), o.mass_energy(ref),
+                       rtol=1e-12, atol=1e-14), seed
+    w = ref + 0.01 * np.sin(np.arange(ref.size))
+    assert np.allclose(op.l2_error(dev(w), 0.37), o.l2_error(w, 0.37),
+                       rtol=1e-12, atol=0), seed
+    r = np.random.default_rng(90 + seed).standard_normal(o.n_dofs)
+    d = dev(r)
+    op.apply_inverse_mass_matrix(d)
+    assert rel_diff(host(d), o.inverse_mass(r)) < TOL, seed
