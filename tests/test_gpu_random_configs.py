@@ -224,3 +224,32 @@ def test_random_config_auxiliary(gpu_lib, seed):
     d = dev(r)
     op.apply_inverse_mass_matrix(d)
     assert rel_diff(host(d), o.inverse_mass(r)) < TOL, seed
+
+
+def fp32_draw(kw):
+    """FP32 operators: advection, and SIPG on axis-aligned cells."""
+    k = kw.get("operator_kind", 0)
+    return k == 0 or (k == 1 and kw["perturbation"] == 0)
+
+
+FP32_SEEDS = [s for s in SEEDS if fp32_draw(draw(s))]
+
+
+@pytest.mark.parametrize("seed", FP32_SEEDS)
+def test_random_config_fp32_apply_error(gpu_lib, seed):
+    """The FP32 apply stays within a small multiple of the error that
+    rounding the input to FP32 causes through the FP64 operator."""
+    kw = dict(draw(seed), precision=1)
+    print(f"seed {seed}: {kw}", flush=True)
+    c = Cfg(**kw)
+    op = AdvectionOperation(c.dg())
+    c.precision = 0
+    o = c.oracle_handle()
+    u = np.random.default_rng(70 + seed).standard_normal(o.n_dofs)
+    dst = op.initialize_dof_vector()
+    op.apply(dev(u).float(), dst, 0.1)
+    ref = o.apply(u, 0.1)
+    floor = rel_diff(o.apply(u.astype(np.float32).astype(np.float64), 0.1),
+                     ref)
+    err = rel_diff(host(dst), ref)
+    assert err < max(20 * floor, 2e-6), (seed, err, floor)
