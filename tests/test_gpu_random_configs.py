@@ -12,6 +12,7 @@ import pytest
 import torch
 
 from advection_miniapp_amd import AdvectionOperation
+from advection_miniapp_amd import _lib as L
 from helpers import Cfg, rel_diff
 
 pytestmark = pytest.mark.gpu
@@ -253,3 +254,28 @@ def test_random_config_fp32_apply_error(gpu_lib, seed):
                      ref)
     err = rel_diff(host(dst), ref)
     assert err < max(20 * floor, 2e-6), (seed, err, floor)
+
+
+@pytest.mark.parametrize("seed", SEEDS[::2])
+def test_random_config_time_loop_in_one_call(gpu_lib, seed):
+    """dg_lsrk_time_steps with graph replay forced equals the same steps
+    taken one by one, bit for bit (time-dependent operators and boundary
+    faces take the per-step path inside the call)."""
+    from advection_miniapp_amd import LowStorageRungeKuttaIntegrator
+    kw = draw(seed)
+    print(f"seed {seed}: {kw}", flush=True)
+    op = AdvectionOperation(Cfg(**kw).dg())
+    op.set_step_launch(2)
+    integ = LowStorageRungeKuttaIntegrator(L.DG_LSRK_5_4)
+    u0 = op.initialize_dof_vector()
+    op.project_initial(u0, 0.0)
+    dt, n = 1e-3, 2 * L.DG_GRAPH_STEPS + 3
+    ue, e1, e2 = u0.clone(), op.initialize_dof_vector(), \
+        op.initialize_dof_vector()
+    for s in range(n):
+        integ.perform_time_step(op, s * dt, dt, ue, e1, e2)
+    ug, g1, g2 = u0.clone(), op.initialize_dof_vector(), \
+        op.initialize_dof_vector()
+    integ.perform_time_steps(op, 0.0, dt, n, ug, g1, g2)
+    torch.cuda.synchronize()
+    assert torch.equal(ug, ue), seed
