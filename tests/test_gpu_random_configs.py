@@ -109,6 +109,11 @@ def test_random_config_apply_and_stage(gpu_lib, seed):
     print(f"seed {seed}: {kw}", flush=True)
     c = Cfg(**kw)
     op, o = AdvectionOperation(c.dg()), c.oracle_handle()
+    # mesh and index data bit-exact
+    assert np.array_equal(op.cell_coordinates(), o.cell_coordinates())
+    assert np.array_equal(op.neighbors(), o.neighbors())
+    assert np.array_equal(op.cell_vertices(), o.cell_vertices())
+    assert op.min_vertex_distance == o.min_vertex_distance()
     rng = np.random.default_rng(seed)
     n = o.n_dofs * c.n_components
     u = rng.standard_normal(n)
