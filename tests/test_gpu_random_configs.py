@@ -78,7 +78,7 @@ def draw(seed):
     return kw
 
 
-SEEDS = list(range(32))
+SEEDS = list(range(64))
 
 
 def dev(a):
