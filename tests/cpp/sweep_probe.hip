@@ -1,12 +1,13 @@
-// Throughput of the 1D sweep at degree 4 (5-point lines, FP64): the
-// even-odd VALU products of the operator kernels (eo_apply, one line per
-// thread) against FP64 matrix cores (v_mfma_f64_16x16x4f64, three lines
-// block-diagonal per matrix column, 48 lines per wavefront). Both apply
-// S and S^-1 alternately `reps` times to every line of a vector the size
-// of the 64^3, degree-4 apply (32,768,000 values); reps = 0 is the memory
-// time alone. Between two MFMA sweeps the result is turned back into the
-// B-operand layout through a wavefront-private LDS tile, as a sweep in
-// another direction would need.
+// Throughput of the 1D sweep at degree k (n = k + 1 points per line,
+// FP64): the even-odd VALU products of the operator kernels (eo_apply, one
+// line per thread) against FP64 matrix cores (v_mfma_f64_16x16x4f64, LPB =
+// 16 / n lines block-diagonal per matrix column, 16 LPB lines per
+// wavefront: 3 at degree 4, 2 at degree 7, where two 8 x 8 blocks fill the
+// 16 rows). Both apply S and S^-1 alternately `reps` times to every line
+// of a vector of 32,768,000 values (the size of the 64^3, degree-4 apply);
+// reps = 0 is the memory time alone. Between two MFMA sweeps the result is
+// turned back into the B-operand layout through a wavefront-private LDS
+// tile, as a sweep in another direction would need.
 //
 // The lane layout of the MFMA operands is not assumed: a first kernel
 // multiplies one-hot B operands with lane-numbered A operands, the host
@@ -14,7 +15,8 @@
 // entries every lane holds and hands these tables to the sweep kernel.
 // The results of both sweep kernels are compared on the host.
 //
-//     sweep_probe [reps ...]    default 0 8 32 128
+//     sweep_probe [-k degree] [reps ...]    degree 1..7, default 4;
+//                                            reps default 0 8 32 128
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -22,6 +24,7 @@
 #include <cstdlib>
 #include <map>
 #include <set>
+#include <string>
 #include <vector>
 
 #include "dev_basis.hpp"
@@ -31,7 +34,6 @@ using namespace dgops;
 
 namespace
 {
-  constexpr int n = 5;
   typedef double v4d __attribute__((ext_vector_type(4)));
 
   void
@@ -147,6 +149,7 @@ namespace
     return a_seen.size() == 64 && d_seen.size() == 256;
   }
 
+  template <int n>
   __global__ void __launch_bounds__(256)
   valu_sweeps(const DevBasis<double> *__restrict__ B, const double *in,
               double *out, int64_t lines, int reps)
@@ -173,18 +176,20 @@ namespace
       out[l * n + i] = v[i];
   }
 
-  // Abd: [2][16][16] block-diagonal diag(S, S, S) and diag(S^-1, S^-1,
-  // S^-1), zero row and column 15; K-slice s of the product holds columns
-  // 4 s .. 4 s + 3 of A. Column c of the B operand carries lines 3 c ..
-  // 3 c + 2 of the wavefront's 48 (row r = 5 b + j is point j of line
-  // 3 c + b).
+  // Abd: [2][16][16] block-diagonal, LPB copies of S and of S^-1, zero in
+  // the rows and columns from LPB n on; K-slice s of the product holds
+  // columns 4 s .. 4 s + 3 of A. Column c of the B operand carries lines
+  // LPB c .. LPB c + LPB - 1 of the wavefront's 16 LPB (row r = n b + j is
+  // point j of line LPB c + b).
+  template <int n>
   __global__ void __launch_bounds__(256)
   mfma_sweeps(const double *__restrict__ Abd, const Layout *__restrict__ L,
               const double *in, double *out, int64_t lines, int reps)
   {
+    constexpr int LPB = 16 / n;
     __shared__ double tile[4][16 * 16];
     const int     lane = threadIdx.x % 64, wv = threadIdx.x / 64;
-    const int64_t l0   = ((int64_t)blockIdx.x * 4 + wv) * 48;
+    const int64_t l0   = ((int64_t)blockIdx.x * 4 + wv) * 16 * LPB;
     double       *T    = tile[wv];
     const int     ar = L->a_row[lane], ak = L->a_k[lane];
     const int     bk = L->b_k[lane], bc = L->b_col[lane];
@@ -205,8 +210,8 @@ namespace
     for (int s = 0; s < 4; ++s)
       {
         const int     r    = 4 * s + bk;
-        const int64_t line = l0 + 3 * bc + r / 5;
-        b[s] = (r < 15 && line < lines) ? in[line * n + r % 5] : 0.;
+        const int64_t line = l0 + LPB * bc + r / n;
+        b[s] = (r < LPB * n && line < lines) ? in[line * n + r % n] : 0.;
       }
     v4d d = {b[0], b[1], b[2], b[3]};
     for (int rep = 0; rep < reps; ++rep)
@@ -234,10 +239,119 @@ namespace
     for (int s = 0; s < 4; ++s)
       {
         const int     r    = 4 * s + bk;
-        const int64_t line = l0 + 3 * bc + r / 5;
-        if (r < 15 && line < lines)
-          out[line * n + r % 5] = b[s];
+        const int64_t line = l0 + LPB * bc + r / n;
+        if (r < LPB * n && line < lines)
+          out[line * n + r % n] = b[s];
       }
+  }
+} // namespace
+
+namespace
+{
+  // times both sweep forms for n-point lines and compares their results;
+  // returns 0, or 2 if they differ
+  template <int n>
+  int
+  run(const Layout &L, const std::vector<int> &reps_list)
+  {
+    constexpr int LPB   = 16 / n;
+    const int64_t lines = 32768000 / n;
+    const Basis1D basis = make_basis(n);
+    const auto    hb    = make_dev_basis<double>(basis);
+    std::vector<double> Abd(2 * 256, 0.);
+    for (int b = 0; b < LPB; ++b)
+      for (int q = 0; q < n; ++q)
+        for (int i = 0; i < n; ++i)
+          {
+            Abd[(n * b + q) * 16 + n * b + i]       = hb.S[q][i];
+            Abd[256 + (n * b + q) * 16 + n * b + i] = hb.Sinv[q][i];
+          }
+    std::vector<double> h(lines * n);
+    for (int64_t i = 0; i < lines * n; ++i)
+      h[i] = std::sin(0.001 * (double)i);
+    DevBasis<double> *dB;
+    Layout           *dL;
+    double           *dA, *din, *dv, *dm;
+    check(hipMalloc(&dB, sizeof(hb)), "hipMalloc");
+    check(hipMalloc(&dL, sizeof(L)), "hipMalloc");
+    check(hipMalloc(&dA, sizeof(double) * Abd.size()), "hipMalloc");
+    check(hipMalloc(&din, sizeof(double) * h.size()), "hipMalloc");
+    check(hipMalloc(&dv, sizeof(double) * h.size()), "hipMalloc");
+    check(hipMalloc(&dm, sizeof(double) * h.size()), "hipMalloc");
+    check(hipMemcpy(dB, &hb, sizeof(hb), hipMemcpyHostToDevice), "copy");
+    check(hipMemcpy(dL, &L, sizeof(L), hipMemcpyHostToDevice), "copy");
+    check(hipMemcpy(dA, Abd.data(), sizeof(double) * Abd.size(),
+                    hipMemcpyHostToDevice),
+          "copy");
+    check(hipMemcpy(din, h.data(), sizeof(double) * h.size(),
+                    hipMemcpyHostToDevice),
+          "copy");
+    hipEvent_t e0, e1;
+    check(hipEventCreate(&e0), "event");
+    check(hipEventCreate(&e1), "event");
+    const unsigned gv = (unsigned)((lines + 255) / 256);
+    const unsigned gm = (unsigned)((lines + 64 * LPB - 1) / (64 * LPB));
+    int            rc = 0;
+    std::printf("degree %d: %d points per line, %d lines per MFMA block "
+                "(%d of 16 rows), %lld lines\n",
+                n - 1, n, LPB, LPB * n, (long long)lines);
+    for (const int reps : reps_list)
+      {
+        float tv = 0, tm = 0;
+        for (int it = 0; it < 4; ++it)
+          {
+            float t;
+            check(hipEventRecord(e0, 0), "event");
+            hipLaunchKernelGGL((valu_sweeps<n>), dim3(gv), dim3(256), 0, 0,
+                               dB, din, dv, lines, reps);
+            check(hipEventRecord(e1, 0), "event");
+            check(hipEventSynchronize(e1), "sync");
+            check(hipEventElapsedTime(&t, e0, e1), "time");
+            if (it > 0)
+              tv += t / 3;
+            check(hipEventRecord(e0, 0), "event");
+            hipLaunchKernelGGL((mfma_sweeps<n>), dim3(gm), dim3(256), 0, 0,
+                               dA, dL, din, dm, lines, reps);
+            check(hipEventRecord(e1, 0), "event");
+            check(hipEventSynchronize(e1), "sync");
+            check(hipEventElapsedTime(&t, e0, e1), "time");
+            if (it > 0)
+              tm += t / 3;
+          }
+        check(hipGetLastError(), "launch");
+        std::vector<double> rv(h.size()), rm(h.size());
+        check(hipMemcpy(rv.data(), dv, sizeof(double) * h.size(),
+                        hipMemcpyDeviceToHost),
+              "copy");
+        check(hipMemcpy(rm.data(), dm, sizeof(double) * h.size(),
+                        hipMemcpyDeviceToHost),
+              "copy");
+        double diff = 0, mx = 0;
+        for (size_t i = 0; i < h.size(); ++i)
+          {
+            diff = std::fmax(diff, std::fabs(rv[i] - rm[i]));
+            mx   = std::fmax(mx, std::fabs(rv[i]));
+          }
+        // line sweeps per second, and the dense-equivalent rate (2 n^2
+        // flop per line and sweep)
+        const double sweeps = (double)lines * reps;
+        std::printf("reps %3d: VALU even-odd %.3f ms (%.2f G line-sweeps/s, "
+                    "%.1f TFLOP/s dense-eq.), MFMA f64 16x16x4 %.3f ms "
+                    "(%.2f, %.1f), max difference %.2e of %.2e\n",
+                    reps, tv, sweeps / (tv * 1e-3) / 1e9,
+                    2. * n * n * sweeps / (tv * 1e-3) / 1e12, tm,
+                    sweeps / (tm * 1e-3) / 1e9,
+                    2. * n * n * sweeps / (tm * 1e-3) / 1e12, diff, mx);
+        if (!(diff <= 1e-11 * mx))
+          rc = 2;
+      }
+    check(hipFree(dB), "hipFree");
+    check(hipFree(dL), "hipFree");
+    check(hipFree(dA), "hipFree");
+    check(hipFree(din), "hipFree");
+    check(hipFree(dv), "hipFree");
+    check(hipFree(dm), "hipFree");
+    return rc;
   }
 } // namespace
 
@@ -245,10 +359,21 @@ int
 main(int argc, char **argv)
 {
   std::vector<int> reps_list;
+  int              degree = 4;
   for (int i = 1; i < argc; ++i)
-    reps_list.push_back(std::atoi(argv[i]));
+    {
+      if (std::string(argv[i]) == "-k" && i + 1 < argc)
+        degree = std::atoi(argv[++i]);
+      else
+        reps_list.push_back(std::atoi(argv[i]));
+    }
   if (reps_list.empty())
     reps_list = {0, 8, 32, 128};
+  if (degree < 1 || degree > 7)
+    {
+      std::fprintf(stderr, "degree must be 1..7\n");
+      return 1;
+    }
 
   double *dlay;
   check(hipMalloc(&dlay, sizeof(double) * 64 * 64 * 4), "hipMalloc");
@@ -274,93 +399,14 @@ main(int argc, char **argv)
               L.b_col[17], L.d_row[0][0], L.d_col[0][0], L.d_row[0][1],
               L.d_col[0][1], L.d_row[0][2], L.d_col[0][2], L.d_row[0][3],
               L.d_col[0][3], L.d_row[16][0], L.d_col[16][0]);
-
-  const int64_t lines = 32768000 / n;
-  const Basis1D basis = make_basis(n);
-  const auto    hb    = make_dev_basis<double>(basis);
-  std::vector<double> Abd(2 * 256, 0.);
-  for (int b = 0; b < 3; ++b)
-    for (int q = 0; q < n; ++q)
-      for (int i = 0; i < n; ++i)
-        {
-          Abd[(5 * b + q) * 16 + 5 * b + i]       = hb.S[q][i];
-          Abd[256 + (5 * b + q) * 16 + 5 * b + i] = hb.Sinv[q][i];
-        }
-  std::vector<double> h(lines * n);
-  for (int64_t i = 0; i < lines * n; ++i)
-    h[i] = std::sin(0.001 * (double)i);
-  DevBasis<double> *dB;
-  Layout           *dL;
-  double           *dA, *din, *dv, *dm;
-  check(hipMalloc(&dB, sizeof(hb)), "hipMalloc");
-  check(hipMalloc(&dL, sizeof(L)), "hipMalloc");
-  check(hipMalloc(&dA, sizeof(double) * Abd.size()), "hipMalloc");
-  check(hipMalloc(&din, sizeof(double) * h.size()), "hipMalloc");
-  check(hipMalloc(&dv, sizeof(double) * h.size()), "hipMalloc");
-  check(hipMalloc(&dm, sizeof(double) * h.size()), "hipMalloc");
-  check(hipMemcpy(dB, &hb, sizeof(hb), hipMemcpyHostToDevice), "copy");
-  check(hipMemcpy(dL, &L, sizeof(L), hipMemcpyHostToDevice), "copy");
-  check(hipMemcpy(dA, Abd.data(), sizeof(double) * Abd.size(),
-                  hipMemcpyHostToDevice),
-        "copy");
-  check(hipMemcpy(din, h.data(), sizeof(double) * h.size(),
-                  hipMemcpyHostToDevice),
-        "copy");
-  hipEvent_t e0, e1;
-  check(hipEventCreate(&e0), "event");
-  check(hipEventCreate(&e1), "event");
-  const unsigned gv = (unsigned)((lines + 255) / 256);
-  const unsigned gm = (unsigned)((lines + 191) / 192);
-  int            rc = 0;
-  for (const int reps : reps_list)
+  switch (degree)
     {
-      float tv = 0, tm = 0;
-      for (int it = 0; it < 4; ++it)
-        {
-          float t;
-          check(hipEventRecord(e0, 0), "event");
-          hipLaunchKernelGGL(valu_sweeps, dim3(gv), dim3(256), 0, 0, dB, din,
-                             dv, lines, reps);
-          check(hipEventRecord(e1, 0), "event");
-          check(hipEventSynchronize(e1), "sync");
-          check(hipEventElapsedTime(&t, e0, e1), "time");
-          if (it > 0)
-            tv += t / 3;
-          check(hipEventRecord(e0, 0), "event");
-          hipLaunchKernelGGL(mfma_sweeps, dim3(gm), dim3(256), 0, 0, dA, dL,
-                             din, dm, lines, reps);
-          check(hipEventRecord(e1, 0), "event");
-          check(hipEventSynchronize(e1), "sync");
-          check(hipEventElapsedTime(&t, e0, e1), "time");
-          if (it > 0)
-            tm += t / 3;
-        }
-      check(hipGetLastError(), "launch");
-      std::vector<double> rv(h.size()), rm(h.size());
-      check(hipMemcpy(rv.data(), dv, sizeof(double) * h.size(),
-                      hipMemcpyDeviceToHost),
-            "copy");
-      check(hipMemcpy(rm.data(), dm, sizeof(double) * h.size(),
-                      hipMemcpyDeviceToHost),
-            "copy");
-      double diff = 0, mx = 0;
-      for (size_t i = 0; i < h.size(); ++i)
-        {
-          diff = std::fmax(diff, std::fabs(rv[i] - rm[i]));
-          mx   = std::fmax(mx, std::fabs(rv[i]));
-        }
-      // line sweeps per second, and the dense-equivalent rate (2 n^2 flop
-      // per line and sweep)
-      const double sweeps = (double)lines * reps;
-      std::printf("reps %3d: VALU even-odd %.3f ms (%.2f G line-sweeps/s, "
-                  "%.1f TFLOP/s dense-eq.), MFMA f64 16x16x4 %.3f ms (%.2f, "
-                  "%.1f), max difference %.2e of %.2e\n",
-                  reps, tv, sweeps / (tv * 1e-3) / 1e9,
-                  2. * n * n * sweeps / (tv * 1e-3) / 1e12, tm,
-                  sweeps / (tm * 1e-3) / 1e9,
-                  2. * n * n * sweeps / (tm * 1e-3) / 1e12, diff, mx);
-      if (!(diff <= 1e-11 * mx))
-        rc = 2;
+      case 1: return run<2>(L, reps_list);
+      case 2: return run<3>(L, reps_list);
+      case 3: return run<4>(L, reps_list);
+      case 4: return run<5>(L, reps_list);
+      case 5: return run<6>(L, reps_list);
+      case 6: return run<7>(L, reps_list);
+      default: return run<8>(L, reps_list);
     }
-  return rc;
 }
