@@ -892,15 +892,15 @@ namespace dgops
     return g.exec;
   }
 
-  struct EventPair
+  struct StepEvents
   {
     hipEvent_t e[3] = {nullptr, nullptr, nullptr};
-    EventPair()
+    StepEvents()
     {
       for (auto &x : e)
         hip_check(hipEventCreate(&x), "step timing event");
     }
-    ~EventPair()
+    ~StepEvents()
     {
       for (auto &x : e)
         if (x)
@@ -924,7 +924,7 @@ namespace dgops
         const hipGraphExec_t exec = step_graph(op, scheme, dt, buf);
         hip_check(hipGraphLaunch(exec, op.stream), "graph launch");
         s += DG_GRAPH_STEPS;
-        EventPair ev;
+        StepEvents ev;
         hip_check(hipEventRecord(ev.e[0], op.stream), "step timing event");
         for (int i = 0; i < DG_GRAPH_STEPS; ++i, ++s)
           lsrk_step(op, scheme, t + s * dt, dt, buf);

@@ -118,6 +118,7 @@ namespace dgops
     // SIPG on perturbed cells: face geometry of the face above every cell
     // in every direction, built on first use (sipg.hip)
     DevBuf d_sipg_faces;
+    bool   sipg_faces_ready = false;
 
     // implicit state
     double time      = 0;

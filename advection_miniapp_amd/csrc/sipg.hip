@@ -16,7 +16,7 @@ namespace dgops
            hipStream_t stream)
     {
       using L = SipgLay<dim, n>;
-      if (!op.d_sipg_faces.ptr)
+      if (!op.sipg_faces_ready)
         {
           const int64_t entries = n_cells * dim * L::nf;
           op.d_sipg_faces.alloc(sizeof(double) * entries * L::nt);
@@ -24,6 +24,7 @@ namespace dgops
                              dim3((entries + 255) / 256), dim3(256), 0,
                              stream, P, B, op.d_sipg_faces.as<double>());
           hip_check(hipGetLastError(), "SIPG face table kernel");
+          op.sipg_faces_ready = true;
         }
       const int64_t blocks = (n_cells + L::CPB - 1) / L::CPB;
       hipLaunchKernelGGL((sipg_general_kernel<dim, n>), dim3(blocks),
