@@ -4,6 +4,13 @@
  * expected result in the parity tests of the GPU library and as the CPU
  * baseline of bench.py.
  * Nothing in the product links or calls this library.
+ *
+ * Parity unpinned: the mesh's cell order and DoF numbering
+ * (or_cell_coordinates, or_neighbors, or_cell_vertices: the refine_global
+ * child order, lexicographic DoFs per cell) follow a restated rule that the
+ * tests check, but nothing here can compare them with deal.II's DoFHandler
+ * itself. Every other function is checked
+ * against closed forms, known answers, invariants or convergence orders.
  */
 #ifndef DG_ORACLE_H
 #define DG_ORACLE_H
