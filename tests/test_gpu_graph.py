@@ -28,6 +28,16 @@ CASES = {
     "3d_burgers_k3": dict(dim=3, degree=3, n_components=3, operator_kind=2,
                           refine=2, lo=[-1, -1, -1], hi=[1, 1, 1],
                           solution_kind=6),
+    # SIPG: the fused kernel on axis-aligned cells; on perturbed cells
+    # three launches, one of them reading the face table built on first use
+    "3d_k3_sipg": dict(dim=3, degree=3, refine=2, velocity=[1.0, 0.5, 0.25],
+                       operator_kind=1, viscosity=1e-2, penalty_factor=16,
+                       solution_kind=2, solution_param=[2 * np.pi, 1e-2]),
+    "3d_k3_sipg_pert": dict(dim=3, degree=3, refine=2, perturbation=0.1,
+                            seed=7, velocity=[1.0, 0.5, 0.25],
+                            operator_kind=1, viscosity=5e-2,
+                            penalty_factor=16, solution_kind=2,
+                            solution_param=[2 * np.pi, 5e-2]),
     # time-dependent speed, and inflow boundaries: steps launched one by one
     "2d_k4_vortex_time": dict(dim=2, degree=4, refine=3, perturbation=0.1,
                               velocity_kind=2, time_factor=1,
@@ -36,7 +46,8 @@ CASES = {
                             periodic=[0, 1, 0], velocity=[1.1, 0.15, -0.05],
                             solution_kind=1),
 }
-AUTONOMOUS = {"2d_k3_config1", "3d_k4_pert_cellular", "3d_burgers_k3"}
+AUTONOMOUS = {"2d_k3_config1", "3d_k4_pert_cellular", "3d_burgers_k3",
+              "3d_k3_sipg", "3d_k3_sipg_pert"}
 
 
 def run_eager(op, integ, u, r1, r2, t, dt, n):
@@ -90,6 +101,34 @@ def test_time_steps_equal_single_steps(gpu_lib, name, mode):
                                 else 1)
     else:
         assert info["mode"] == 0 and info["eager_ms"] == 0
+
+
+@pytest.mark.parametrize("name", ["3d_k3_sipg_pert", "3d_k4_pert_cellular"])
+def test_time_steps_as_first_call_of_an_operator(gpu_lib, name):
+    """An operator whose first call is a multi-step call with graph replay
+    forced: the tables built on first use (geometry, the SIPG face table)
+    are built by the call's eager first step, before any capture; the
+    result equals single steps on a second operator of the same
+    configuration, bit for bit."""
+    c = Cfg(**CASES[name])
+    print(f"case {name}: first call with graph replay", flush=True)
+    integ = LowStorageRungeKuttaIntegrator(1)
+    dt, n = 1e-3, 2 * G + 3
+    opg = AdvectionOperation(c.dg())
+    opg.set_step_launch(2)
+    ug = opg.initialize_dof_vector()
+    opg.project_initial(ug, 0.0)
+    u0 = ug.clone()
+    g1, g2 = opg.initialize_dof_vector(), opg.initialize_dof_vector()
+    integ.perform_time_steps(opg, 0.0, dt, n, ug, g1, g2)
+    torch.cuda.synchronize()
+    ope = AdvectionOperation(c.dg())
+    ue, e1, e2 = vectors(ope, u0)
+    run_eager(ope, integ, ue, e1, e2, 0.0, dt, n)
+    torch.cuda.synchronize()
+    assert opg.step_launch()["mode"] == 2
+    assert not torch.equal(ug, u0)
+    assert torch.equal(ug, ue), name
 
 
 def test_graph_replays_between_eager_work(gpu_lib):
