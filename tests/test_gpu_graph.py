@@ -38,6 +38,15 @@ CASES = {
                             operator_kind=1, viscosity=5e-2,
                             penalty_factor=16, solution_kind=2,
                             solution_param=[2 * np.pi, 5e-2]),
+    # Gauss-Lobatto switches: the same kernels with GL tables, and for the
+    # mass matrix alone the two-pass stage (integrate, inverse mass, update)
+    "3d_k3_gl_quad": dict(dim=3, degree=3, refine=2, perturbation=0.1,
+                          velocity_kind=1, solution_kind=3,
+                          solution_param=[0.5, 0.5, 0.5, 0.1], use_gl_quad=1),
+    "3d_k3_gl_quad_mass": dict(dim=3, degree=3, refine=2, perturbation=0.1,
+                               velocity_kind=1, solution_kind=3,
+                               solution_param=[0.5, 0.5, 0.5, 0.1],
+                               use_gl_quad_mass=1),
     # time-dependent speed, and inflow boundaries: steps launched one by one
     "2d_k4_vortex_time": dict(dim=2, degree=4, refine=3, perturbation=0.1,
                               velocity_kind=2, time_factor=1,
@@ -47,7 +56,8 @@ CASES = {
                             solution_kind=1),
 }
 AUTONOMOUS = {"2d_k3_config1", "3d_k4_pert_cellular", "3d_burgers_k3",
-              "3d_k3_sipg", "3d_k3_sipg_pert"}
+              "3d_k3_sipg", "3d_k3_sipg_pert", "3d_k3_gl_quad",
+              "3d_k3_gl_quad_mass"}
 
 
 def run_eager(op, integ, u, r1, r2, t, dt, n):
